@@ -1,0 +1,26 @@
+"""CPU oracle for the PRT-LB hot path.  TEST INFRASTRUCTURE ONLY.
+
+This package restates, in plain NumPy, the numerics of the reference
+simulator (shahram444/PRT-DeepONet-GUI, files ``3D/tools/prtlb_2d.py``,
+``3D/tools/prtlb_3d.py`` and ``3D/tools/settings_and_units.py``).  It exists
+to CHECK the HIP implementation in ``prt-deeponet-gui_amd/prtlb``; it is never
+what is measured or shipped.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` may import it.
+
+Pinning (how each function is shown to be right, independently of itself):
+
+* ``stokes``          -- golden vectors written by the reference itself
+                         (tests/golden), Poiseuille closed form, TRT==BGK at
+                         tau=0.9330127, mass conservation, two-step
+                         bounce-back identity, D2Q9==D3Q19 on a slit.
+* ``solve_adr``       -- golden vectors from the reference (bit-exact float32),
+                         plug flow exp(-Da), maximum principle, sealed half
+                         stays exactly zero, uniform field invariant.
+* geometry helpers    -- golden vectors from the reference (scipy.ndimage).
+* ``check_physics``   -- golden complaint lists from the reference.
+* ``permeability``    -- slit closed form k = (2H^3+H)/(24(H+2)).
+* ``outlet_concentration`` -- plug flow exp(-Da).
+* ``sorption_reversible`` -- parity unpinned against the reference (the
+                         reference has no reversible sorption); pinned by the
+                         well-mixed closed form C(t)=Cinf+(C0-Cinf)e^{-(ka+kd)t}.
+"""
