@@ -1,0 +1,196 @@
+/*
+ * prtlb.h -- C ABI of libprtlb_hip.so, the MI355X (gfx950) PRT-LB hot path.
+ *
+ * The reference (shahram444/PRT-DeepONet-GUI) has no FFI: its hot path is two
+ * NumPy functions, stokes_d2q9/stokes_d3q19 and solve_adr.  Each entry point
+ * below replaces the NumPy loop it names; the Python package `prtlb`
+ * (prt-deeponet-gui_amd/prtlb) binds them with ctypes and keeps the
+ * reference's Python signatures.
+ *
+ * Conventions
+ *   - every function returns 0 on success or a negative PRT_E* code; the
+ *     message for the calling thread is prt_last_error().
+ *   - all array arguments are DEVICE pointers owned by the caller (PyTorch
+ *     tensors in the Python host); the library never allocates, frees or
+ *     keeps them.  `stream` is a hipStream_t (NULL = the null stream); every
+ *     launch is asynchronous on it.
+ *   - grids are C-ordered (x slowest, last axis fastest), exactly the NumPy
+ *     layout of the reference's (nx, ny[, nz]) arrays.  A 2D grid is passed
+ *     with nz = 1.
+ *   - populations are structure-of-arrays: population i of node n lives at
+ *     f[i * n_nodes + n].
+ */
+#ifndef PRTLB_H
+#define PRTLB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define PRT_OK 0
+#define PRT_E_ARG -1     /* bad argument (shape, null pointer, enum) */
+#define PRT_E_HIP -2     /* a HIP runtime call failed */
+#define PRT_E_RANGE -3   /* size beyond what the kernel indexes */
+
+/* population storage (all collide arithmetic in FP64 / FP32 / FP32) */
+#define PRT_STORE_FP64 0 /* f_i, double: the reference's own arithmetic */
+#define PRT_STORE_FP32 1 /* f_i - w_i, float */
+#define PRT_STORE_FP16 2 /* (f_i - w_i) * (s / w_i), IEEE half */
+
+/* prt_lbm_step flags */
+#define PRT_LBM_SKIP_SOLID 1 /* solid nodes are neither read nor written */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *prt_last_error(void);
+
+/* Library version string. */
+const char *prt_version(void);
+
+/* sizeof(prt_adr_params) as compiled into the library (binding check). */
+size_t prt_adr_params_size(void);
+
+/* ------------------------------------------------------------------ Stokes */
+
+/* Rest state f_i = w_i (zero deviation in the shifted storages) for n_nodes
+ * nodes.  Replaces `f = np.ones(...) * w` (3D/tools/prtlb_3d.py:403,
+ * 3D/tools/prtlb_2d.py:427). */
+int prt_lbm_fill_rest(int q, int storage, int64_t n_nodes, void *f,
+                      void *stream);
+
+/* One fused TRT iteration: pull-stream from f_src, collide, Guo force,
+ * solid-node reversal, write f_dst.  Replaces the loop body at
+ * 3D/tools/prtlb_3d.py:427-445 (D3Q19, q = 19) and
+ * 3D/tools/prtlb_2d.py:449-472 (D2Q9, q = 9).
+ *   solid     n_nodes bytes, nonzero = solid (g != PORE)
+ *   omega_p   1/tau; omega_m from the magic parameter (prt_trt_minus)
+ *   force     body force along +x
+ *   fp16_scale  s of PRT_STORE_FP16 (ignored otherwise)
+ *   partials  if non-NULL: per-block FP64 partial sums of |u_x| over fluid
+ *             nodes (the FlowConvergence quantity, 3D/tools/prtlb_3d.py:444)
+ *             are written here; prt_lbm_partials_count() gives the length.
+ *   f_dst must not alias f_src.  With PRT_LBM_SKIP_SOLID, f_dst must hold
+ *   the state of the step before f_src (the A/B pair) -- bounce-back then
+ *   reads a fluid node's own two-step-old slot instead of the solid node. */
+int prt_lbm_step(int q, int storage, int nx, int ny, int nz,
+                 const uint8_t *solid, const void *f_src, void *f_dst,
+                 double omega_p, double omega_m, double force,
+                 double fp16_scale, int flags, double *partials, void *stream);
+
+/* Number of partial sums prt_lbm_step writes for this grid. */
+int64_t prt_lbm_partials_count(int q, int nx, int ny, int nz);
+
+/* `steps` fused iterations alternating f_a -> f_b -> f_a ...; the last one
+ * writes f_b if steps is odd, else f_a.  Partials (if non-NULL) are
+ * produced by the last iteration only. */
+int prt_lbm_run(int q, int storage, int nx, int ny, int nz,
+                const uint8_t *solid, void *f_a, void *f_b, int steps,
+                double omega_p, double omega_m, double force,
+                double fp16_scale, int flags, double *partials, void *stream);
+
+/* Deterministic FP64 sum of n doubles (fixed order) into *out (device). */
+int prt_sum_f64(const double *x, int64_t n, double *out, void *stream);
+
+/* Velocity from the post-collision state f_last (pull, density, Guo-shifted
+ * momentum, zero in solid).  Replaces 3D/tools/prtlb_3d.py:451-455 and
+ * 3D/tools/prtlb_2d.py:481-485.  v64 (3, n) double and/or v32 (3, n) float,
+ * either may be NULL; for 2D the third row is 0.  partials (device, may be
+ * NULL, prt_lbm_partials_count() long) receives per-block FP64 sums of u_x
+ * over every node, for the Darcy permeability (reduce with prt_sum_f64). */
+int prt_lbm_velocity(int q, int storage, int nx, int ny, int nz,
+                     const uint8_t *solid, const void *f_last, double force,
+                     double fp16_scale, double *v64, float *v32,
+                     double *partials, void *stream);
+
+/* Sum of u_x over each x-plane of v64 (3, nx, ny, nz): plane[nx]. */
+int prt_plane_flux(const double *v64, int nx, int ny, int nz, double *plane,
+                   void *stream);
+
+/* --------------------------------------------------------------- transport */
+
+/* Scalar parameters of one transport solve, built by the host exactly as
+ * 3D/tools/prtlb_3d.py:831-1018 builds them.  Every value the float32 path
+ * uses is a float32 number carried in a double (lossless). */
+typedef struct prt_adr_params {
+  int32_t nd;            /* 2 or 3 */
+  int32_t ns;            /* chemicals, 1..8 */
+  int32_t biotic_on;     /* 3D/tools/prtlb_3d.py:1288 */
+  int32_t coupled;       /* biomass_coupled && ns > 3 && c_init[3] > 0 */
+  int32_t abio_index;    /* abiotic reactant, or -1 when the abiotic rate is
+                            the scalar zero (ka <= 0 or ns <= index) */
+  int32_t abio_target;   /* abiotic_index if < ns, else -1: where -Ra lands */
+  int32_t abio_product;  /* product index (!= reactant, < ns) or -1 */
+  int32_t surface_only;  /* abiotic_surface_only */
+  int32_t sorption;      /* 1: reversible sorption (build-defined) between
+                            mobile sorb_c and immobile sorb_s */
+  int32_t sorb_c, sorb_s;
+  int32_t write_solid;   /* 1: also write the zeros outside the pore */
+  double dt, k, ka, kd;
+  double ks_donor, ks_acceptor;  /* <= 0 means first order */
+  double alpha_dt;       /* float(alpha) * dt, as NumPy rounds it */
+  double neg_alpha;      /* -alpha */
+  double alpha_pos;      /* 1 if alpha > 0 */
+  double c2;             /* 1 - yield (ns > 3) or 1, rounded once */
+  double yield_;
+  double c3_init;
+  double k_sorb, k_desorb; /* reversible sorption rates */
+  int32_t mobile[8];     /* 1 = transported */
+  int32_t pinned[8];     /* 1 = mobile with a feed: held at fed[j] on x = 0 */
+  double fed[8];
+  double dface[8][4];    /* harmonic face diffusivity, [j][2*bf_self + bf_nb] */
+} prt_adr_params;
+
+/* One explicit step of the flux-form MUSCL-minmod transport plus reactions:
+ * inlet pin (x = 0, fed mobile species), transport of the mobile rows,
+ * C + dt (T + R), clip at zero in the pore, zero outside.  Replaces step()
+ * at 3D/tools/prtlb_3d.py:1392-1413 with transport :1182-1208 and reaction
+ * :1288-1367 fused in.  Grid (nx, ny, nz) with nz = 1 for 2D.
+ *   flags  n bytes: bit0 pore-or-biofilm (solute may be here), bit1 biofilm
+ *   u      (nd, n): velocity normalised by its axial pore mean, 0 off the
+ *          pore (3D/tools/prtlb_3d.py:860-865)
+ *   c_in, c_out  (ns, n); c_out must not alias c_in.  Unless write_solid,
+ *          voxels outside the pore are not written (keep them zero).
+ * The float32 path is bit-exact with the reference's float32 NumPy. */
+int prt_adr_step(const prt_adr_params *p, int nx, int ny, int nz,
+                 const uint8_t *flags, const float *u, const float *c_in,
+                 float *c_out, void *stream);
+
+/* Same step in FP64 (the build's own double-precision variant). */
+int prt_adr_step_f64(const prt_adr_params *p, int nx, int ny, int nz,
+                     const uint8_t *flags, const double *u,
+                     const double *c_in, double *c_out, void *stream);
+
+/* `steps` float32 steps alternating c_a -> c_b -> c_a ...; the last one
+ * writes c_b if steps is odd, else c_a. */
+int prt_adr_run(const prt_adr_params *p, int nx, int ny, int nz,
+                const uint8_t *flags, const float *u, float *c_a, float *c_b,
+                int steps, void *stream);
+
+/* ------------------------------------------------------ NumPy-order sums */
+
+/* Host-side plan of NumPy's float32 np.add.reduce order for n contiguous
+ * values: 8192-element buffers summed left to right, each buffer by pairwise
+ * recursion (n/2 rounded down to a multiple of 8 per split, leaves of at
+ * most 128 with eight accumulators).  With plan == NULL only the sizes are
+ * returned; the plan needs 2*n_leaves + 2*n_nodes + n_levels + 1 +
+ * ceil(n/8192) int32 (HOST memory).  Used for the pore means of
+ * 3D/tools/prtlb_3d.py:1500 (C[j][m].mean()). */
+int prt_pairwise_plan(int64_t n, int32_t *plan, int64_t capacity,
+                      int64_t *n_leaves, int64_t *n_nodes, int64_t *n_levels);
+
+/* out[r] = the float32 NumPy sum of c[r * ld + idx[k]], k < n, for each of
+ * `rows` rows, evaluated on the device with the plan (copied to the device)
+ * and scratch of rows * (n_leaves + n_nodes) floats. */
+int prt_pairwise_sum_f32(const float *c, int64_t ld, int rows,
+                         const int32_t *idx, int64_t n, const int32_t *plan,
+                         int64_t n_leaves, int64_t n_nodes, int64_t n_levels,
+                         float *scratch, float *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PRTLB_H */

@@ -416,7 +416,7 @@ class AdrSetup:
 
 
 def adr_setup(mat, vel, pe, da, n_species=2, chem=None, da_abio=0.0,
-              real=np.float32):
+              real=np.float32, sorption=None):
     R = np.dtype(real).type
     S = AdrSetup()
     mat = np.asarray(mat)
@@ -476,8 +476,15 @@ def adr_setup(mat, vel, pe, da, n_species=2, chem=None, da_abio=0.0,
         k_lin = max(k_lin, float(k) / float(ch.ks_donor))
     if ns > 1 and ch.ks_acceptor > 0:
         k_lin = max(k_lin, float(k) / float(ch.ks_acceptor))
-    # NEP 50: the Python-float partial sum becomes the array dtype at `+ ka`
-    dt = R(0.4 / (umax + 2.0 * nd * dmax + k_lin + ka + kd + 1e-30))
+    k_sorb = k_des = R(0.0)
+    if sorption:
+        # build-defined reversible sorption: its two rates join the limit
+        k_sorb = R(float(sorption.get("da_sorb", 0.0)) / l_ref)
+        k_des = R(float(sorption.get("da_desorb", 0.0)) / l_ref)
+        dt = R(0.4 / (umax + 2.0 * nd * dmax + k_lin + ka + kd + k_sorb + k_des + 1e-30))
+    else:
+        # NEP 50: the Python-float partial sum becomes the array dtype at `+ ka`
+        dt = R(0.4 / (umax + 2.0 * nd * dmax + k_lin + ka + kd + 1e-30))
 
     N = m.size
     nb = clamp_table(m.shape)
@@ -489,6 +496,16 @@ def adr_setup(mat, vel, pe, da, n_species=2, chem=None, da_abio=0.0,
             nface += (~mf[nb[ax][-1]]).astype(R)
             nface += (~mf[nb[ax][1]]).astype(R)
         surf = np.where(mf, nface / float(2 * nd), 0.0).astype(R)
+    surf_sorb = None
+    if sorption:
+        if getattr(ch, "abiotic_surface_only", False):
+            nface = np.zeros(N, R)
+            for ax in range(nd):
+                nface += (~mf[nb[ax][-1]]).astype(R)
+                nface += (~mf[nb[ax][1]]).astype(R)
+            surf_sorb = np.where(mf, nface / float(2 * nd), 0.0).astype(R)
+        else:
+            surf_sorb = np.ones(N, R)
     moving = [j for j in range(ns) if not ch.immobile[j]]
     js = list(range(ns)) if len(moving) == ns else moving
     faces = []
@@ -525,7 +542,8 @@ def adr_setup(mat, vel, pe, da, n_species=2, chem=None, da_abio=0.0,
         u_scale=u_scale, l_ref=l_ref, D=D, k=k, ka=ka, kd=kd, dt=dt,
         pe_grid=pe_grid, dmax=dmax, umax=umax, surf=surf, faces=faces,
         divu=divu, moving=moving, inlet=inlet,
-        fed=np.asarray(ch.inlet, R), dfield=dfield))
+        fed=np.asarray(ch.inlet, R), dfield=dfield, sorption=sorption,
+        k_sorb=k_sorb, k_des=k_des, surf_sorb=surf_sorb))
     return S
 
 
@@ -606,6 +624,16 @@ def adr_reaction(S, C):
         mk = getattr(ch, "abiotic_product_index", None)
         if mk is not None and mk < ns and mk != j:
             out[mk] = out[mk] + Ra
+    if S.sorption:
+        # build-defined reversible sorption C <-> S (no reference counterpart)
+        c, s = int(S.sorption.get("c", 0)), int(S.sorption.get("s", 1))
+        fw = (S.k_sorb * C[c]) * S.surf_sorb
+        fw = np.minimum(fw, 0.5 * C[c] / dt)
+        bk = S.k_des * C[s]
+        bk = np.minimum(bk, 0.5 * C[s] / dt)
+        net = fw - bk
+        out[c] = out[c] - net
+        out[s] = out[s] + net
     return out
 
 
@@ -627,10 +655,12 @@ def adr_step(S, C):
 
 def solve_adr(mat, vel, pe, da, n_t, n_species=2, steps=None, tol=2e-3,
               max_steps=100000, info=None, ratio=1.12, chem=None, da_abio=0.0,
-              on_progress=None, real=np.float32, return_state=False):
+              on_progress=None, real=np.float32, return_state=False,
+              sorption=None):
     """Transport solve; 3D/tools/prtlb_3d.py:781-1595.  Returns (out, t_norm)
-    and with return_state also the final (ns, *grid) field."""
-    S = adr_setup(mat, vel, pe, da, n_species, chem, da_abio, real)
+    and with return_state also the final (ns, *grid) field.  `sorption`
+    (build-defined): dict(da_sorb, da_desorb, c, s) for reversible sorption."""
+    S = adr_setup(mat, vel, pe, da, n_species, chem, da_abio, real, sorption)
     R, ns, mf = S.R, S.ns, S.mf
     shape = S.m.shape
     C = adr_initial(S)

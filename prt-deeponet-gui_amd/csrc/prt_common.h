@@ -1,0 +1,43 @@
+// Shared helpers of the PRT-LB HIP library: error reporting and launch checks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/prtlb.h"
+
+namespace prt {
+
+// Thread-local message of the last failure (read by prt_last_error()).
+void set_error(const char *fmt, ...);
+
+inline int hip_fail(hipError_t e, const char *what) {
+  set_error("%s: %s", what, hipGetErrorString(e));
+  return PRT_E_HIP;
+}
+
+// Check the launch that was just issued.
+inline int launch_status(const char *what) {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? PRT_OK : hip_fail(e, what);
+}
+
+// Block-wide FP64 sum of one value per thread (blockDim.x * blockDim.y <=
+// 1024, a multiple of 64).  Wavefront reduction with shuffles, then LDS.
+__device__ inline double block_sum(double v, double *lds) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int t = threadIdx.y * blockDim.x + threadIdx.x;
+  const int nw = (blockDim.x * blockDim.y) >> 6;
+  if ((t & 63) == 0) lds[t >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (t == 0)
+    for (int k = 0; k < nw; ++k) s += lds[k];
+  return s;  // valid in thread 0
+}
+
+}  // namespace prt

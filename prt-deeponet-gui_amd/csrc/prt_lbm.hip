@@ -1,0 +1,592 @@
+// prt_lbm.hip -- fused TRT lattice-Boltzmann Stokes kernels for gfx950.
+//
+// One thread per node, pull scheme over a structure-of-arrays A/B pair:
+// every launch gathers the Q populations that stream INTO a node (periodic on
+// every axis), collides them (TRT with the magic parameter, first-order Guo
+// force with omega-), applies the solid-node reversal and writes the node's
+// Q post-collision populations once.  Each population is read once and
+// written once per iteration: 2*Q*sizeof(pop) bytes per node update, the
+// whole of the algorithmic traffic (HBM-bound, 1.3 flop/B in FP64; no MFMA).
+//
+// The reference's iteration (3D/tools/prtlb_3d.py:427-445) is
+//   collide at every node -> solid nodes take f_opp(i) -> np.roll stream.
+// Storing post-collision populations and pulling at the start of the next
+// launch is the same sequence cut at a different place: f^t(x) =
+// fp^{t-1}(x - e_i) is gathered, fp^t(x) is written.
+//
+// A 2D D2Q9 grid (nx, ny) is indexed as the 3D grid (nx, 1, ny): same bytes
+// (C order), one kernel.  The D2Q9 velocity (e_x, e_y) becomes (e_x, 0, e_y).
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "prt_common.h"
+
+namespace prt {
+
+template <int Q>
+struct Lat;
+
+// D3Q19, ordering of 3D/tools/prtlb_3d.py:370-375 (required for population
+// level parity with the reference's state).
+template <>
+struct Lat<19> {
+  static constexpr int cx[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+  static constexpr int cy[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+  static constexpr int cz[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+  static constexpr int opp[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+  static constexpr int cls[19] = {0, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};
+};
+// D2Q9 of 3D/tools/prtlb_2d.py:391-394, stored as (x, -, y) -> (x, y=0, z).
+template <>
+struct Lat<9> {
+  static constexpr int cx[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+  static constexpr int cy[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  static constexpr int cz[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+  static constexpr int opp[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+  static constexpr int cls[9] = {0, 1, 1, 1, 1, 2, 2, 2, 2};
+};
+
+template <int Q>
+__host__ __device__ constexpr double weight(int i) {
+  return Q == 19 ? (Lat<19>::cls[i] == 0 ? 1.0 / 3.0 : Lat<19>::cls[i] == 1 ? 1.0 / 18.0 : 1.0 / 36.0)
+                 : (Lat<9>::cls[i] == 0 ? 4.0 / 9.0 : Lat<9>::cls[i] == 1 ? 1.0 / 9.0 : 1.0 / 36.0);
+}
+
+struct LbmArgs {
+  const uint8_t *solid;
+  const void *src;
+  void *dst;
+  double *partials;
+  int X, Y, Z;
+  long long N;
+  double omp, omm, g, half_g;
+  double guo[19];     // w_i (1 - omm/2) 3 g e_ix, formed as the reference does
+  float guo32[19];
+  float ld16[19];     // w_i / s     (FP16 load scale)
+  float st16[19];     // s / w_i     (FP16 store scale)
+};
+
+// ---------------------------------------------------------------- storage
+template <int ST>
+struct Pop;
+template <>
+struct Pop<PRT_STORE_FP64> {
+  using T = double;
+};
+template <>
+struct Pop<PRT_STORE_FP32> {
+  using T = float;
+};
+template <>
+struct Pop<PRT_STORE_FP16> {
+  using T = __half;
+};
+
+// Gather index of the node that streams into (x, y, z) along e_i.
+struct Nbr {
+  int xm, xp, ym, yp, zm, zp, x, y, z, Y, Z;
+  __device__ inline int at(int ex, int ey, int ez) const {
+    const int sx = ex > 0 ? xm : (ex < 0 ? xp : x);
+    const int sy = ey > 0 ? ym : (ey < 0 ? yp : y);
+    const int sz = ez > 0 ? zm : (ez < 0 ? zp : z);
+    return (sx * Y + sy) * Z + sz;
+  }
+};
+
+__device__ inline Nbr make_nbr(int x, int y, int z, int X, int Y, int Z) {
+  Nbr b;
+  b.x = x; b.y = y; b.z = z; b.Y = Y; b.Z = Z;
+  b.xm = x == 0 ? X - 1 : x - 1;
+  b.xp = x == X - 1 ? 0 : x + 1;
+  b.ym = y == 0 ? Y - 1 : y - 1;
+  b.yp = y == Y - 1 ? 0 : y + 1;
+  b.zm = z == 0 ? Z - 1 : z - 1;
+  b.zp = z == Z - 1 ? 0 : z + 1;
+  return b;
+}
+
+// Load the Q populations streaming into node n (SKIP: from the node's own
+// two-step-old opposite slot where the upstream node is solid).
+template <int Q, int ST, bool SKIP, typename R>
+__device__ inline void gather(const LbmArgs &a, const Nbr &nb, int n, R *f) {
+  using T = typename Pop<ST>::T;
+  const T *src = static_cast<const T *>(a.src);
+  const T *own = static_cast<const T *>(a.dst);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int s = (i == 0) ? n : nb.at(Lat<Q>::cx[i], Lat<Q>::cy[i], Lat<Q>::cz[i]);
+    T v;
+    if (SKIP && i != 0 && a.solid[s])
+      v = own[(size_t)Lat<Q>::opp[i] * a.N + n];
+    else
+      v = src[(size_t)i * a.N + s];
+    if constexpr (ST == PRT_STORE_FP16)
+      f[i] = __half2float(v) * a.ld16[i];
+    else
+      f[i] = (R)v;
+  }
+}
+
+template <int Q, int ST, typename R>
+__device__ inline void scatter(const LbmArgs &a, int n, const R *fp) {
+  using T = typename Pop<ST>::T;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    if constexpr (ST == PRT_STORE_FP16) {
+      unsigned short *dst = static_cast<unsigned short *>(a.dst);
+      const unsigned short v = __half_as_ushort(__float2half_rn(fp[i] * a.st16[i]));
+      __builtin_nontemporal_store(v, dst + (size_t)i * a.N + n);
+    } else {
+      T *dst = static_cast<T *>(a.dst);
+      __builtin_nontemporal_store((T)fp[i], dst + (size_t)i * a.N + n);
+    }
+  }
+}
+
+// FP64 collision in the reference's own evaluation order (contraction off):
+// rho = sum f, j = einsum(e, f), u = (j + g/2 e_x)/rho masked, feq, TRT with
+// f+/- = (f +/- f_opp)/2, then + Guo.  3D/tools/prtlb_3d.py:142-154, 427-440.
+template <int Q>
+__device__ inline void collide_ref(const LbmArgs &a, bool solid, const double *f,
+                                   double *fp, double &ux_out) {
+#pragma clang fp contract(off)
+  double rho = f[0];
+#pragma unroll
+  for (int i = 1; i < Q; ++i) rho += f[i];
+  double jx = 0.0, jy = 0.0, jz = 0.0;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    if (Lat<Q>::cx[i]) jx += (double)Lat<Q>::cx[i] * f[i];
+    if (Lat<Q>::cy[i]) jy += (double)Lat<Q>::cy[i] * f[i];
+    if (Lat<Q>::cz[i]) jz += (double)Lat<Q>::cz[i] * f[i];
+  }
+  double ux = (jx + a.half_g) / rho;
+  double uy = jy / rho;
+  double uz = jz / rho;
+  if (solid) ux = uy = uz = 0.0;
+  ux_out = ux;
+  const double usq = (ux * ux + uy * uy) + uz * uz;
+  double feq[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const double eu = ((double)Lat<Q>::cx[i] * ux + (double)Lat<Q>::cy[i] * uy) +
+                      (double)Lat<Q>::cz[i] * uz;
+    feq[i] = (weight<Q>(i) * rho) * (((1.0 + 3.0 * eu) + (4.5 * eu) * eu) - 1.5 * usq);
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int o = Lat<Q>::opp[i];
+    const double f_p = 0.5 * (f[i] + f[o]);
+    const double f_m = 0.5 * (f[i] - f[o]);
+    const double e_p = 0.5 * (feq[i] + feq[o]);
+    const double e_m = 0.5 * (feq[i] - feq[o]);
+    double v = (f[i] - a.omp * (f_p - e_p)) - a.omm * (f_m - e_m);
+    v = v + a.guo[i];
+    fp[i] = solid ? f[o] : v;
+  }
+}
+
+// FP32 collision on deviations g_i = f_i - w_i (the shifted storages).
+template <int Q>
+__device__ inline void collide_dev(const LbmArgs &a, bool solid, const float *gq,
+                                   float *out, double &ux_out) {
+  float dr = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    dr += gq[i];
+    if (Lat<Q>::cx[i]) jx += Lat<Q>::cx[i] > 0 ? gq[i] : -gq[i];
+    if (Lat<Q>::cy[i]) jy += Lat<Q>::cy[i] > 0 ? gq[i] : -gq[i];
+    if (Lat<Q>::cz[i]) jz += Lat<Q>::cz[i] > 0 ? gq[i] : -gq[i];
+  }
+  const float rho = 1.f + dr;
+  const float irho = 1.f / rho;
+  float ux = (jx + (float)a.half_g) * irho, uy = jy * irho, uz = jz * irho;
+  if (solid) ux = uy = uz = 0.f;
+  ux_out = ux;
+  const float usq = ux * ux + uy * uy + uz * uz;
+  const float omp = (float)a.omp, omm = (float)a.omm;
+  {
+    const float eq0 = (float)weight<Q>(0) * (dr - 1.5f * rho * usq);
+    out[0] = solid ? gq[0] : gq[0] - omp * (gq[0] - eq0);
+  }
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    const int o = Lat<Q>::opp[i];
+    if (o < i) continue;
+    const float w = (float)weight<Q>(i);
+    const float eu = (float)Lat<Q>::cx[i] * ux + (float)Lat<Q>::cy[i] * uy + (float)Lat<Q>::cz[i] * uz;
+    const float sym = w * (dr + rho * (4.5f * eu * eu - 1.5f * usq));
+    const float asym = w * rho * 3.f * eu;
+    const float gp = 0.5f * (gq[i] + gq[o]);
+    const float gm = 0.5f * (gq[i] - gq[o]);
+    const float A = omp * (gp - sym);
+    const float B = omm * (gm - asym);
+    const float vi = gq[i] - A - B + a.guo32[i];
+    const float vo = gq[o] - A + B + a.guo32[o];
+    out[i] = solid ? gq[o] : vi;
+    out[o] = solid ? gq[i] : vo;
+  }
+}
+
+template <int Q, int ST, bool SKIP, bool RED>
+__global__ __launch_bounds__(256) void lbm_step_kernel(LbmArgs a) {
+  __shared__ double lds[16];
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int x = blockIdx.z;
+  const bool inside = z < a.Z && y < a.Y;
+  double contrib = 0.0;
+  if (inside) {
+    const int n = (x * a.Y + y) * a.Z + z;
+    const bool solid = a.solid[n] != 0;
+    if (!(SKIP && solid)) {
+      const Nbr nb = make_nbr(x, y, z, a.X, a.Y, a.Z);
+      double ux;
+      if constexpr (ST == PRT_STORE_FP64) {
+        double f[Q], fp[Q];
+        gather<Q, ST, SKIP>(a, nb, n, f);
+        collide_ref<Q>(a, solid, f, fp, ux);
+        scatter<Q, ST>(a, n, fp);
+      } else {
+        float f[Q], fp[Q];
+        gather<Q, ST, SKIP>(a, nb, n, f);
+        collide_dev<Q>(a, solid, f, fp, ux);
+        scatter<Q, ST>(a, n, fp);
+      }
+      if (RED && !solid) contrib = fabs(ux);
+    }
+  }
+  if (RED) {
+    const double s = block_sum(contrib, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+      a.partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+  }
+}
+
+// Macroscopic field of the final state (3D/tools/prtlb_3d.py:451-455).
+template <int Q, int ST>
+__global__ __launch_bounds__(256) void lbm_velocity_kernel(LbmArgs a, double *v64, float *v32) {
+#pragma clang fp contract(off)
+  __shared__ double lds[16];
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int x = blockIdx.z;
+  double contrib = 0.0;
+  if (z < a.Z && y < a.Y) {
+    const int n = (x * a.Y + y) * a.Z + z;
+    const bool solid = a.solid[n] != 0;
+    const Nbr nb = make_nbr(x, y, z, a.X, a.Y, a.Z);
+    double u[3];
+    if constexpr (ST == PRT_STORE_FP64) {
+      double f[Q];
+      gather<Q, ST, false>(a, nb, n, f);
+      double rho = f[0];
+#pragma unroll
+      for (int i = 1; i < Q; ++i) rho += f[i];
+      double j[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        if (Lat<Q>::cx[i]) j[0] += (double)Lat<Q>::cx[i] * f[i];
+        if (Lat<Q>::cy[i]) j[1] += (double)Lat<Q>::cy[i] * f[i];
+        if (Lat<Q>::cz[i]) j[2] += (double)Lat<Q>::cz[i] * f[i];
+      }
+      u[0] = (j[0] + a.half_g) / rho;
+      u[1] = j[1] / rho;
+      u[2] = j[2] / rho;
+    } else {
+      float f[Q];
+      gather<Q, ST, false>(a, nb, n, f);
+      double dr = 0.0, j[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        dr += (double)f[i];
+        if (Lat<Q>::cx[i]) j[0] += (double)Lat<Q>::cx[i] * (double)f[i];
+        if (Lat<Q>::cy[i]) j[1] += (double)Lat<Q>::cy[i] * (double)f[i];
+        if (Lat<Q>::cz[i]) j[2] += (double)Lat<Q>::cz[i] * (double)f[i];
+      }
+      const double rho = 1.0 + dr;
+      u[0] = (j[0] + a.half_g) / rho;
+      u[1] = j[1] / rho;
+      u[2] = j[2] / rho;
+    }
+    if (solid) u[0] = u[1] = u[2] = 0.0;
+    // a 2D grid carries its second velocity component in the z slot
+    const double o0 = u[0], o1 = (Q == 9) ? u[2] : u[1], o2 = (Q == 9) ? 0.0 : u[2];
+    if (v64) {
+      v64[n] = o0;
+      v64[a.N + n] = o1;
+      v64[2 * a.N + n] = o2;
+    }
+    if (v32) {
+      v32[n] = (float)o0;
+      v32[a.N + n] = (float)o1;
+      v32[2 * a.N + n] = (float)o2;
+    }
+    contrib = o0;
+  }
+  if (a.partials) {
+    const double s = block_sum(contrib, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+      a.partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+  }
+}
+
+template <int Q, int ST>
+__global__ void fill_rest_kernel(void *f, long long N) {
+  using T = typename Pop<ST>::T;
+  T *p = static_cast<T *>(f);
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < N * Q; k += stride) {
+    const int i = (int)(k / N);
+    if constexpr (ST == PRT_STORE_FP64)
+      p[k] = weight<Q>(i);
+    else
+      p[k] = (T)0.0f;
+  }
+}
+
+// Deterministic sum: one block, fixed per-thread stride order, fixed combine.
+__global__ __launch_bounds__(1024) void sum_f64_kernel(const double *x, long long n, double *out) {
+  __shared__ double part[1024];
+  double s = 0.0;
+  for (long long k = threadIdx.x; k < n; k += blockDim.x) s += x[k];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = part[0];
+}
+
+// One block per x-plane: sum of v64[0][x, ...] in a fixed order.
+__global__ __launch_bounds__(256) void plane_flux_kernel(const double *v, int plane_n, double *out) {
+  __shared__ double part[256];
+  const double *p = v + (size_t)blockIdx.x * plane_n;
+  double s = 0.0;
+  for (int k = threadIdx.x; k < plane_n; k += blockDim.x) s += p[k];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = part[0];
+}
+
+// ------------------------------------------------------------------ host side
+struct Geom {
+  int X, Y, Z;
+  long long N;
+  dim3 block, grid;
+};
+
+static int make_geom(int q, int nx, int ny, int nz, Geom &g) {
+  if (q != 9 && q != 19) {
+    set_error("q must be 9 (D2Q9) or 19 (D3Q19), got %d", q);
+    return PRT_E_ARG;
+  }
+  if (nx < 1 || ny < 1 || nz < 1) {
+    set_error("grid dimensions must be positive (%d, %d, %d)", nx, ny, nz);
+    return PRT_E_ARG;
+  }
+  if (q == 9 && nz != 1) {
+    set_error("D2Q9 takes a 2D grid: pass nz = 1 (got %d)", nz);
+    return PRT_E_ARG;
+  }
+  if (q == 9) {
+    g.X = nx; g.Y = 1; g.Z = ny;
+  } else {
+    g.X = nx; g.Y = ny; g.Z = nz;
+  }
+  g.N = (long long)nx * ny * nz;
+  if (g.N * q >= (1LL << 31) || g.X > 65535) {
+    set_error("grid of %lld nodes exceeds the 32-bit node index of this build", g.N);
+    return PRT_E_RANGE;
+  }
+  int bz = 64;
+  while (bz < g.Z && bz < 256) bz <<= 1;
+  const int by = 256 / bz;
+  g.block = dim3(bz, by, 1);
+  g.grid = dim3((g.Z + bz - 1) / bz, (g.Y + by - 1) / by, g.X);
+  return PRT_OK;
+}
+
+static void fill_args(LbmArgs &a, int q, const Geom &g, const uint8_t *solid, const void *src, void *dst,
+                      double omp, double omm, double force, double s16, double *partials) {
+  std::memset(&a, 0, sizeof(a));
+  a.solid = solid; a.src = src; a.dst = dst; a.partials = partials;
+  a.X = g.X; a.Y = g.Y; a.Z = g.Z; a.N = g.N;
+  a.omp = omp; a.omm = omm; a.g = force; a.half_g = 0.5 * force;
+  for (int i = 0; i < q; ++i) {
+    const double w = q == 19 ? weight<19>(i) : weight<9>(i);
+    const int ex = q == 19 ? Lat<19>::cx[i] : Lat<9>::cx[i];
+    // same association as the reference: w * (1 - 0.5 om_m) * 3.0 * gf * ex
+    volatile double t = w * (1 - 0.5 * omm);
+    t = t * 3.0;
+    t = t * force;
+    a.guo[i] = t * (double)ex;
+    a.guo32[i] = (float)a.guo[i];
+    a.ld16[i] = (float)(w / s16);
+    a.st16[i] = (float)(s16 / w);
+  }
+}
+
+template <int Q, int ST>
+static void launch_step_t(const Geom &g, const LbmArgs &a, bool skip, bool red, hipStream_t st) {
+  if (skip) {
+    if (red) lbm_step_kernel<Q, ST, true, true><<<g.grid, g.block, 0, st>>>(a);
+    else lbm_step_kernel<Q, ST, true, false><<<g.grid, g.block, 0, st>>>(a);
+  } else {
+    if (red) lbm_step_kernel<Q, ST, false, true><<<g.grid, g.block, 0, st>>>(a);
+    else lbm_step_kernel<Q, ST, false, false><<<g.grid, g.block, 0, st>>>(a);
+  }
+}
+
+static int launch_step(int q, int storage, const Geom &g, const LbmArgs &a, bool skip, bool red,
+                       hipStream_t st) {
+  switch (q * 4 + storage) {
+    case 19 * 4 + PRT_STORE_FP64: launch_step_t<19, PRT_STORE_FP64>(g, a, skip, red, st); break;
+    case 19 * 4 + PRT_STORE_FP32: launch_step_t<19, PRT_STORE_FP32>(g, a, skip, red, st); break;
+    case 19 * 4 + PRT_STORE_FP16: launch_step_t<19, PRT_STORE_FP16>(g, a, skip, red, st); break;
+    case 9 * 4 + PRT_STORE_FP64: launch_step_t<9, PRT_STORE_FP64>(g, a, skip, red, st); break;
+    case 9 * 4 + PRT_STORE_FP32: launch_step_t<9, PRT_STORE_FP32>(g, a, skip, red, st); break;
+    case 9 * 4 + PRT_STORE_FP16: launch_step_t<9, PRT_STORE_FP16>(g, a, skip, red, st); break;
+    default:
+      set_error("unknown storage %d", storage);
+      return PRT_E_ARG;
+  }
+  return launch_status("prt_lbm_step launch");
+}
+
+static int check_omega(double omp, double omm, double s16, int storage) {
+  if (!(omp > 0.0 && omp < 2.0) || !(omm > 0.0 && omm < 2.0)) {
+    set_error("relaxation rates must lie in (0, 2): omega+ %g, omega- %g (tau must exceed 0.5)", omp, omm);
+    return PRT_E_ARG;
+  }
+  if (storage == PRT_STORE_FP16 && !(s16 > 0.0)) {
+    set_error("fp16_scale must be positive");
+    return PRT_E_ARG;
+  }
+  return PRT_OK;
+}
+
+}  // namespace prt
+
+using namespace prt;
+
+extern "C" int prt_lbm_fill_rest(int q, int storage, int64_t n_nodes, void *f, void *stream) {
+  if (!f || n_nodes < 1 || (q != 9 && q != 19)) {
+    set_error("prt_lbm_fill_rest: bad arguments");
+    return PRT_E_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int blocks = 2048;
+  switch (q * 4 + storage) {
+    case 19 * 4 + PRT_STORE_FP64: fill_rest_kernel<19, PRT_STORE_FP64><<<blocks, 256, 0, st>>>(f, n_nodes); break;
+    case 19 * 4 + PRT_STORE_FP32: fill_rest_kernel<19, PRT_STORE_FP32><<<blocks, 256, 0, st>>>(f, n_nodes); break;
+    case 19 * 4 + PRT_STORE_FP16: fill_rest_kernel<19, PRT_STORE_FP16><<<blocks, 256, 0, st>>>(f, n_nodes); break;
+    case 9 * 4 + PRT_STORE_FP64: fill_rest_kernel<9, PRT_STORE_FP64><<<blocks, 256, 0, st>>>(f, n_nodes); break;
+    case 9 * 4 + PRT_STORE_FP32: fill_rest_kernel<9, PRT_STORE_FP32><<<blocks, 256, 0, st>>>(f, n_nodes); break;
+    case 9 * 4 + PRT_STORE_FP16: fill_rest_kernel<9, PRT_STORE_FP16><<<blocks, 256, 0, st>>>(f, n_nodes); break;
+    default:
+      set_error("unknown storage %d", storage);
+      return PRT_E_ARG;
+  }
+  return launch_status("prt_lbm_fill_rest launch");
+}
+
+extern "C" int64_t prt_lbm_partials_count(int q, int nx, int ny, int nz) {
+  Geom g;
+  if (make_geom(q, nx, ny, nz, g) != PRT_OK) return -1;
+  return (int64_t)g.grid.x * g.grid.y * g.grid.z;
+}
+
+extern "C" int prt_lbm_step(int q, int storage, int nx, int ny, int nz, const uint8_t *solid, const void *f_src,
+                            void *f_dst, double omega_p, double omega_m, double force, double fp16_scale,
+                            int flags, double *partials, void *stream) {
+  Geom g;
+  int rc = make_geom(q, nx, ny, nz, g);
+  if (rc) return rc;
+  if ((rc = check_omega(omega_p, omega_m, fp16_scale, storage))) return rc;
+  if (!solid || !f_src || !f_dst || f_src == f_dst) {
+    set_error("prt_lbm_step: null or aliased population buffers");
+    return PRT_E_ARG;
+  }
+  LbmArgs a;
+  fill_args(a, q, g, solid, f_src, f_dst, omega_p, omega_m, force, fp16_scale, partials);
+  return launch_step(q, storage, g, a, (flags & PRT_LBM_SKIP_SOLID) != 0, partials != nullptr,
+                     (hipStream_t)stream);
+}
+
+extern "C" int prt_lbm_run(int q, int storage, int nx, int ny, int nz, const uint8_t *solid, void *f_a, void *f_b,
+                           int steps, double omega_p, double omega_m, double force, double fp16_scale, int flags,
+                           double *partials, void *stream) {
+  Geom g;
+  int rc = make_geom(q, nx, ny, nz, g);
+  if (rc) return rc;
+  if ((rc = check_omega(omega_p, omega_m, fp16_scale, storage))) return rc;
+  if (!solid || !f_a || !f_b || f_a == f_b || steps < 0) {
+    set_error("prt_lbm_run: null or aliased population buffers, or negative steps");
+    return PRT_E_ARG;
+  }
+  LbmArgs a;
+  fill_args(a, q, g, solid, f_a, f_b, omega_p, omega_m, force, fp16_scale, nullptr);
+  for (int s = 0; s < steps; ++s) {
+    a.src = (s & 1) ? f_b : f_a;
+    a.dst = (s & 1) ? f_a : f_b;
+    const bool red = partials && s == steps - 1;
+    a.partials = red ? partials : nullptr;
+    rc = launch_step(q, storage, g, a, (flags & PRT_LBM_SKIP_SOLID) != 0, red, (hipStream_t)stream);
+    if (rc) return rc;
+  }
+  return PRT_OK;
+}
+
+extern "C" int prt_sum_f64(const double *x, int64_t n, double *out, void *stream) {
+  if (!x || !out || n < 0) {
+    set_error("prt_sum_f64: bad arguments");
+    return PRT_E_ARG;
+  }
+  sum_f64_kernel<<<1, 1024, 0, (hipStream_t)stream>>>(x, n, out);
+  return launch_status("prt_sum_f64 launch");
+}
+
+extern "C" int prt_lbm_velocity(int q, int storage, int nx, int ny, int nz, const uint8_t *solid,
+                                const void *f_last, double force, double fp16_scale, double *v64, float *v32,
+                                double *partials, void *stream) {
+  Geom g;
+  int rc = make_geom(q, nx, ny, nz, g);
+  if (rc) return rc;
+  if (!solid || !f_last || (storage == PRT_STORE_FP16 && !(fp16_scale > 0))) {
+    set_error("prt_lbm_velocity: bad arguments");
+    return PRT_E_ARG;
+  }
+  LbmArgs a;
+  fill_args(a, q, g, solid, f_last, nullptr, 1.0, 1.0, force, fp16_scale > 0 ? fp16_scale : 1.0, partials);
+  hipStream_t st = (hipStream_t)stream;
+  switch (q * 4 + storage) {
+    case 19 * 4 + PRT_STORE_FP64: lbm_velocity_kernel<19, PRT_STORE_FP64><<<g.grid, g.block, 0, st>>>(a, v64, v32); break;
+    case 19 * 4 + PRT_STORE_FP32: lbm_velocity_kernel<19, PRT_STORE_FP32><<<g.grid, g.block, 0, st>>>(a, v64, v32); break;
+    case 19 * 4 + PRT_STORE_FP16: lbm_velocity_kernel<19, PRT_STORE_FP16><<<g.grid, g.block, 0, st>>>(a, v64, v32); break;
+    case 9 * 4 + PRT_STORE_FP64: lbm_velocity_kernel<9, PRT_STORE_FP64><<<g.grid, g.block, 0, st>>>(a, v64, v32); break;
+    case 9 * 4 + PRT_STORE_FP32: lbm_velocity_kernel<9, PRT_STORE_FP32><<<g.grid, g.block, 0, st>>>(a, v64, v32); break;
+    case 9 * 4 + PRT_STORE_FP16: lbm_velocity_kernel<9, PRT_STORE_FP16><<<g.grid, g.block, 0, st>>>(a, v64, v32); break;
+    default:
+      set_error("unknown storage %d", storage);
+      return PRT_E_ARG;
+  }
+  return launch_status("prt_lbm_velocity launch");
+}
+
+extern "C" int prt_plane_flux(const double *v64, int nx, int ny, int nz, double *plane, void *stream) {
+  if (!v64 || !plane || nx < 1 || ny < 1 || nz < 1) {
+    set_error("prt_plane_flux: bad arguments");
+    return PRT_E_ARG;
+  }
+  plane_flux_kernel<<<nx, 256, 0, (hipStream_t)stream>>>(v64, ny * nz, plane);
+  return launch_status("prt_plane_flux launch");
+}

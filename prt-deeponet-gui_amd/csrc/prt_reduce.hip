@@ -1,0 +1,175 @@
+// prt_reduce.hip -- NumPy-ordered float32 sums on the GPU.
+//
+// The transport solve stops on per-species pore means computed as
+// float(C[j][m].mean()) (3D/tools/prtlb_3d.py:1500): a float32 np.add.reduce
+// over the compacted pore voxels.  NumPy evaluates that sum by recursive
+// halving (split n/2 rounded down to a multiple of 8) down to blocks of at
+// most 128, each summed with eight interleaved accumulators.  Reproducing
+// that tree makes the stop step -- and so every stored snapshot -- identical
+// to the reference's, not merely close.
+//
+// NumPy feeds a long reduction to that loop in buffers of 8192 elements and
+// accumulates the buffer sums in order (acc = 0; acc += pairwise(chunk)), so
+// the full order is: pairwise trees over consecutive 8192-element chunks,
+// then a left-to-right chain over the chunk sums.
+//
+// The trees depend on n only, so they are planned once on the host: the leaf
+// ranges, then the internal nodes grouped by height (children before
+// parents), then the chunk roots.  The device evaluates leaves in parallel,
+// then one workgroup per row combines level by level and runs the chain.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "prt_common.h"
+
+namespace prt {
+
+struct PlanBuilder {
+  std::vector<int32_t> leaf_start, leaf_len;
+  std::vector<int32_t> left, right, height;  // internal nodes (ids >= n_leaves after renumbering)
+  // NumPy's recursion over a[s, s+n): returns (node id, height); leaves get
+  // ids 0..L-1 in order, internal nodes temporary ids -(k+1)
+  std::pair<long long, int> walk(long long s, long long n) {
+    if (n <= 128) {
+      leaf_start.push_back((int32_t)s);
+      leaf_len.push_back((int32_t)n);
+      return {(long long)leaf_start.size() - 1, 0};
+    }
+    long long n2 = n / 2;
+    n2 -= n2 % 8;
+    auto l = walk(s, n2);
+    auto r = walk(s + n2, n - n2);
+    left.push_back((int32_t)l.first);
+    right.push_back((int32_t)r.first);
+    const int h = 1 + (l.second > r.second ? l.second : r.second);
+    height.push_back(h);
+    return {-(long long)left.size(), h};
+  }
+};
+
+// Leaf sums: 8 accumulators, pairwise combine, sequential tail; below 8
+// elements a plain sequential sum from zero.  Mirrors NumPy's float32
+// pairwise_sum leaf (numpy/_core/src/umath/loops_utils.h.src).
+__global__ void pw_leaf_kernel(const float *c, long long ld, int rows, const int32_t *idx, const int32_t *lstart,
+                               const int32_t *llen, long long nleaves, long long stride_vals, float *vals) {
+#pragma clang fp contract(off)
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nleaves * rows) return;
+  const int row = (int)(t / nleaves);
+  const long long leaf = t - (long long)row * nleaves;
+  const float *a = c + (size_t)row * ld;
+  const int s = lstart[leaf], n = llen[leaf];
+  float res;
+  if (n < 8) {
+    res = 0.f;
+    for (int i = 0; i < n; ++i) res += a[idx[s + i]];
+  } else {
+    float r[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = a[idx[s + j]];
+    int i = 8;
+    for (; i < n - (n % 8); i += 8)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) r[j] += a[idx[s + i + j]];
+    res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    for (; i < n; ++i) res += a[idx[s + i]];
+  }
+  vals[(size_t)row * stride_vals + leaf] = res;
+}
+
+__global__ __launch_bounds__(1024) void pw_combine_kernel(const int32_t *left, const int32_t *right,
+                                                          const int32_t *level_off, int nlevels,
+                                                          const int32_t *roots, int nchunks,
+                                                          long long nleaves, long long stride_vals, float *vals,
+                                                          float *out) {
+#pragma clang fp contract(off)
+  float *v = vals + (size_t)blockIdx.x * stride_vals;
+  for (int h = 0; h < nlevels; ++h) {
+    for (int k = level_off[h] + threadIdx.x; k < level_off[h + 1]; k += blockDim.x)
+      v[nleaves + k] = v[left[k]] + v[right[k]];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    // the buffer sums, left to right from zero
+    float acc = 0.f;
+    for (int k = 0; k < nchunks; ++k) acc = acc + v[roots[k]];
+    out[blockIdx.x] = acc;
+  }
+}
+
+}  // namespace prt
+
+using namespace prt;
+
+// Plan layout (int32): [L leaf_start][L leaf_len][I left][I right][H+1 level_off]
+// [C chunk roots], with internal nodes renumbered so that level h occupies
+// [level_off[h], level_off[h+1]) and children index the value array
+// (leaves 0..L-1, internal node k at L + k).  C = ceil(n / 8192).
+extern "C" int prt_pairwise_plan(int64_t n, int32_t *plan, int64_t capacity, int64_t *n_leaves,
+                                 int64_t *n_nodes, int64_t *n_levels) {
+  if (n < 1 || n >= (1LL << 31) || !n_leaves || !n_nodes || !n_levels) {
+    set_error("prt_pairwise_plan: n must be in [1, 2^31)");
+    return PRT_E_ARG;
+  }
+  PlanBuilder b;
+  const long long kChunk = 8192;  // NumPy's reduction buffer, in elements
+  std::vector<long long> roots;
+  for (long long s0 = 0; s0 < n; s0 += kChunk) roots.push_back(b.walk(s0, n - s0 < kChunk ? n - s0 : kChunk).first);
+  const long long C = (long long)roots.size();
+  const long long L = (long long)b.leaf_start.size();
+  const long long I = (long long)b.left.size();
+  int H = 0;
+  for (int h : b.height) H = h > H ? h : H;
+  *n_leaves = L;
+  *n_nodes = I;
+  *n_levels = H;
+  const long long need = 2 * L + 2 * I + H + 1 + C;
+  if (!plan) return PRT_OK;  // size query
+  if (capacity < need) {
+    set_error("prt_pairwise_plan: plan buffer holds %lld ints, %lld needed", (long long)capacity, need);
+    return PRT_E_ARG;
+  }
+  // order internal nodes by height (stable: post-order within a level)
+  std::vector<int32_t> count(H + 2, 0), newid(I);
+  for (long long k = 0; k < I; ++k) count[b.height[k]]++;
+  std::vector<int32_t> off(H + 2, 0);
+  for (int h = 1; h <= H; ++h) off[h] = off[h - 1] + count[h];  // level h-1 in slot h-1
+  std::vector<int32_t> fill(off.begin(), off.end());
+  for (long long k = 0; k < I; ++k) newid[k] = fill[b.height[k] - 1]++;
+  auto remap = [&](int32_t id) -> int32_t { return id >= 0 ? id : (int32_t)(L + newid[-(long long)id - 1]); };
+  int32_t *ls = plan, *ll = plan + L, *lf = plan + 2 * L, *rt = plan + 2 * L + I, *lo = plan + 2 * L + 2 * I;
+  for (long long k = 0; k < L; ++k) {
+    ls[k] = b.leaf_start[k];
+    ll[k] = b.leaf_len[k];
+  }
+  for (long long k = 0; k < I; ++k) {
+    lf[newid[k]] = remap(b.left[k]);
+    rt[newid[k]] = remap(b.right[k]);
+  }
+  for (int h = 0; h <= H; ++h) lo[h] = off[h];
+  int32_t *rs = lo + H + 1;
+  for (long long k = 0; k < C; ++k) rs[k] = remap((int32_t)roots[k]);
+  return PRT_OK;
+}
+
+extern "C" int prt_pairwise_sum_f32(const float *c, int64_t ld, int rows, const int32_t *idx, int64_t n,
+                                    const int32_t *plan, int64_t n_leaves, int64_t n_nodes, int64_t n_levels,
+                                    float *scratch, float *out, void *stream) {
+  if (!c || !idx || !plan || !scratch || !out || rows < 1 || n_leaves < 1 || n < 1) {
+    set_error("prt_pairwise_sum_f32: bad arguments");
+    return PRT_E_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const long long L = n_leaves, I = n_nodes;
+  const long long stride = L + I;
+  const int32_t *ls = plan, *ll = plan + L, *lf = plan + 2 * L, *rt = plan + 2 * L + I, *lo = plan + 2 * L + 2 * I;
+  const int32_t *roots = lo + n_levels + 1;
+  const int nchunks = (int)((n + 8191) / 8192);
+  const long long work = L * rows;
+  pw_leaf_kernel<<<(unsigned)((work + 255) / 256), 256, 0, st>>>(c, ld, rows, idx, ls, ll, L, stride, scratch);
+  int rc = launch_status("prt_pairwise_sum_f32 leaf launch");
+  if (rc) return rc;
+  pw_combine_kernel<<<rows, 1024, 0, st>>>(lf, rt, lo, (int)n_levels, roots, nchunks, L, stride, scratch, out);
+  return launch_status("prt_pairwise_sum_f32 combine launch");
+}

@@ -1,0 +1,113 @@
+"""ctypes binding of libprtlb_hip.so (declarations in include/prtlb.h).
+
+The library is required: there is no CPU fallback anywhere in `prtlb`.  If it
+is missing or no HIP device is visible, every solver raises.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libprtlb_hip.so")
+HEADER = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "prtlb.h")
+
+PRT_STORE = {"fp64": 0, "fp32": 1, "fp16": 2}
+PRT_LBM_SKIP_SOLID = 1
+
+
+class PrtError(RuntimeError):
+    """A call into libprtlb_hip.so failed."""
+
+
+class AdrParams(ctypes.Structure):
+    """Mirror of prt_adr_params (include/prtlb.h)."""
+    _fields_ = [
+        ("nd", ctypes.c_int32), ("ns", ctypes.c_int32),
+        ("biotic_on", ctypes.c_int32), ("coupled", ctypes.c_int32),
+        ("abio_index", ctypes.c_int32), ("abio_target", ctypes.c_int32),
+        ("abio_product", ctypes.c_int32), ("surface_only", ctypes.c_int32),
+        ("sorption", ctypes.c_int32), ("sorb_c", ctypes.c_int32),
+        ("sorb_s", ctypes.c_int32), ("write_solid", ctypes.c_int32),
+        ("dt", ctypes.c_double), ("k", ctypes.c_double), ("ka", ctypes.c_double),
+        ("kd", ctypes.c_double), ("ks_donor", ctypes.c_double),
+        ("ks_acceptor", ctypes.c_double), ("alpha_dt", ctypes.c_double),
+        ("neg_alpha", ctypes.c_double), ("alpha_pos", ctypes.c_double),
+        ("c2", ctypes.c_double), ("yield_", ctypes.c_double),
+        ("c3_init", ctypes.c_double), ("k_sorb", ctypes.c_double),
+        ("k_desorb", ctypes.c_double),
+        ("mobile", ctypes.c_int32 * 8), ("pinned", ctypes.c_int32 * 8),
+        ("fed", ctypes.c_double * 8), ("dface", (ctypes.c_double * 4) * 8),
+    ]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+
+_SIGS = {
+    "prt_last_error": (ctypes.c_char_p, []),
+    "prt_version": (ctypes.c_char_p, []),
+    "prt_adr_params_size": (ctypes.c_size_t, []),
+    "prt_lbm_fill_rest": (_I, [_I, _I, _I64, _P, _P]),
+    "prt_lbm_step": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _D, _D, _D, _D, _I, _P, _P]),
+    "prt_lbm_partials_count": (_I64, [_I, _I, _I, _I]),
+    "prt_lbm_run": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _I, _D, _D, _D, _D, _I, _P, _P]),
+    "prt_sum_f64": (_I, [_P, _I64, _P, _P]),
+    "prt_lbm_velocity": (_I, [_I, _I, _I, _I, _I, _P, _P, _D, _D, _P, _P, _P, _P]),
+    "prt_plane_flux": (_I, [_P, _I, _I, _I, _P, _P]),
+    "prt_adr_step": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _P]),
+    "prt_adr_step_f64": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _P]),
+    "prt_adr_run": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _I, _P]),
+    "prt_pairwise_plan": (_I, [_I64, _P, _I64, ctypes.POINTER(_I64),
+                               ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+    "prt_pairwise_sum_f32": (_I, [_P, _I64, _I, _P, _I64, _P, _I64, _I64, _I64, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises PrtError if it was never built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PrtError("libprtlb_hip.so is not built (%s); run "
+                           "`python -m prtlb.build` in prt-deeponet-gui_amd"
+                           % LIB_PATH)
+        h = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def call(name, *args):
+    """Call a status-returning entry point; raise PrtError on failure."""
+    rc = getattr(lib(), name)(*args)
+    if rc != 0:
+        msg = lib().prt_last_error().decode(errors="replace")
+        raise PrtError("%s failed (%d): %s" % (name, rc, msg))
+    return rc
+
+
+def header_symbols(path=HEADER):
+    """Function names declared in include/prtlb.h."""
+    src = open(path).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(prt_[a-z0-9_]+)\s*\(", src)))
+
+
+def pairwise_plan(n):
+    """NumPy's float32 pairwise-sum tree for n values (host int32 array)."""
+    L, I, H = _I64(), _I64(), _I64()
+    call("prt_pairwise_plan", n, None, 0, ctypes.byref(L), ctypes.byref(I), ctypes.byref(H))
+    size = 2 * L.value + 2 * I.value + H.value + 1 + (n + 8191) // 8192
+    plan = np.zeros(size, np.int32)
+    call("prt_pairwise_plan", n, plan.ctypes.data, size, ctypes.byref(L),
+         ctypes.byref(I), ctypes.byref(H))
+    return plan, L.value, I.value, H.value

@@ -1,0 +1,193 @@
+"""CPU-only checks of the product's host side: the C ABI loads and exports
+what include/prtlb.h declares, the NumPy-order sum plan, the host set-up of
+the transport step against the oracle's, the snapshot bookkeeping, the
+geometry helpers and check_physics against the reference's golden vectors.
+No kernel is launched here."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import golden
+from oracle import prtlb_oracle as O
+from prtlb import _lib
+from prtlb.adr import adr_host_setup, ladder, pick_snapshots
+import prtlb
+
+
+def test_library_exports_header():
+    L = _lib.lib()
+    syms = _lib.header_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(L, s), s
+    assert L.prt_adr_params_size() == ctypes.sizeof(_lib.AdrParams)
+    assert b"gfx950" in L.prt_version()
+
+
+def _eval_plan(a, plan, L, I, H):
+    """Evaluate the plan on the CPU with float32 arithmetic: trees per
+    8192-element chunk, then the chunk sums left to right from zero."""
+    a = np.asarray(a, np.float32)
+    ls, ll = plan[:L], plan[L:2 * L]
+    lf, rt = plan[2 * L:2 * L + I], plan[2 * L + I:2 * L + 2 * I]
+    v = np.zeros(L + I, np.float32)
+    for k in range(L):
+        seg = a[ls[k]:ls[k] + ll[k]]
+        if len(seg) < 8:
+            r = np.float32(0)
+            for x in seg:
+                r = np.float32(r + x)
+        else:
+            acc = seg[:8].copy()
+            i = 8
+            while i < len(seg) - len(seg) % 8:
+                acc = (acc + seg[i:i + 8]).astype(np.float32)
+                i += 8
+            r = np.float32(np.float32(np.float32(acc[0] + acc[1]) + np.float32(acc[2] + acc[3]))
+                           + np.float32(np.float32(acc[4] + acc[5]) + np.float32(acc[6] + acc[7])))
+            for x in seg[i:]:
+                r = np.float32(r + x)
+        v[k] = r
+    for k in range(I):
+        v[L + k] = np.float32(v[lf[k]] + v[rt[k]])
+    roots = plan[2 * L + 2 * I + H + 1:]
+    assert len(roots) == (len(a) + 8191) // 8192
+    acc = np.float32(0)
+    for r in roots:
+        acc = np.float32(acc + v[r])
+    return acc
+
+
+@pytest.mark.parametrize("n", [1, 7, 8, 100, 128, 129, 1000, 4097, 33333, 100000,
+                               131072, 250001, 1000003])
+def test_pairwise_plan_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    a = (rng.random(n) * rng.choice([1e-3, 1.0, 1e3], n)).astype(np.float32)
+    plan, L, I, H = _lib.pairwise_plan(n)
+    assert _eval_plan(a, plan, L, I, H) == np.add.reduce(a)
+    # levels are ordered children-first
+    lo = plan[2 * L + 2 * I:2 * L + 2 * I + H + 1]
+    assert lo[0] == 0 and lo[-1] == I
+
+
+ADR = ["adr2d_channel4", "adr2d_disks_abio", "adr2d_disks_default4",
+       "adr2d_sorption_irrev", "adr2d_coupled", "adr2d_biofilm", "adr3d_monod",
+       "adr3d_default3_abio"]
+
+
+def _chem(kw, mod):
+    if kw is None:
+        return None
+    fix = {k: ([float("nan") if x == "nan" else x for x in v] if isinstance(v, list)
+               else (float("nan") if v == "nan" else v)) for k, v in kw.items()}
+    return mod.Chemistry(**fix)
+
+
+@pytest.mark.parametrize("name", ADR)
+def test_host_setup_matches_oracle(name):
+    d = golden(name)
+    kw = dict(d["meta"]["kw"])
+    args = (d["mat"], d["vel"], kw["pe"], kw["da"], kw["n_species"])
+    T = adr_host_setup(*args, chem=_chem(d["meta"]["chem"], prtlb),
+                       da_abio=kw.get("da_abio", 0.0))
+    S = O.adr_setup(*args, chem=_chem(d["meta"]["chem"], O), da_abio=kw.get("da_abio", 0.0))
+    assert T.dt == S.dt and T.dt.dtype == np.float32
+    assert (T.D, T.k, T.ka, T.kd) == (S.D, S.k, S.ka, S.kd)
+    assert np.array_equal(T.u_host, np.stack([S.u[a].ravel() for a in range(S.nd)]))
+    assert np.array_equal(T.c_host, O.adr_initial(S))
+    # face diffusivities: the kernel's 2x2 table reproduces every face value
+    p = T.params
+    bf = S.biofilm.ravel().astype(int)
+    for ax, fa in enumerate(S.faces):
+        for r, j in enumerate(S.moving if len(S.moving) < S.ns else range(S.ns)):
+            tab = np.array(p.dface[j], np.float32)
+            lo = np.where(fa["o_lo"], tab[2 * bf + bf[fa["m1"]]], 0).astype(np.float32)
+            assert np.array_equal(lo, fa["d_lo"][r])
+    assert T.surface_voxels == pytest.approx(float(S.surf.sum()), abs=0)
+
+
+@pytest.mark.parametrize("limit,n_t", [(1, 3), (2, 2), (200, 11), (300, 4), (5000, 6), (20000, 9)])
+def test_ladder_and_pick_match_reference_loop(limit, n_t):
+    # the reference's loop (3D/tools/prtlb_3d.py:1488-1546) with a fixed count
+    keep, nxt = [0], 1
+    for n in range(1, limit + 1):
+        if n >= nxt:
+            keep.append(n)
+            nxt = max(n + 1, int(np.ceil(n * 1.12)))
+    if keep[-1] != limit:
+        keep.append(limit)
+    assert ladder(limit, 1.12) == keep
+    ks = np.asarray(keep)
+    want = O.wanted_steps(limit, n_t)
+    used, picked = set(), []
+    for w in want:
+        i = int(np.abs(ks - w).argmin())
+        if i in used:
+            i = O.nearest_unused(i, used, len(ks))
+        used.add(i)
+        picked.append(i)
+    assert pick_snapshots(keep, limit, n_t) == picked
+
+
+def test_geometry_helpers_match_reference():
+    d = golden("geometry")
+    mt = d["meta"]
+    gk, n = prtlb.keep_inlet_connected(d["g"])
+    assert np.array_equal(gk, d["kept"]) and n == mt["removed"]
+    g1, n1 = prtlb.coat_with_biofilm(gk, 3, 1)
+    g2, n2 = prtlb.coat_with_biofilm(gk, 4, 2)
+    assert np.array_equal(g1, d["coat1"]) and n1 == mt["n1"]
+    assert np.array_equal(g2, d["coat2"]) and n2 == mt["n2"]
+    assert prtlb.assert_finite_distance(np.zeros((3, 3)), np.full((3, 3), 2)) is None
+    bad = np.zeros((3, 3)); bad[1, 1] = np.inf
+    assert "not finite" in prtlb.assert_finite_distance(bad, np.full((3, 3), 2))
+
+
+def test_generators_hit_porosity_and_are_seeded():
+    g = prtlb.disk_pack((128, 128), 6.0, 0.6, seed=0, connect=False)
+    phi = float((g == prtlb.PORE).mean())
+    assert 0.55 < phi <= 0.60
+    assert np.array_equal(g, prtlb.disk_pack((128, 128), 6.0, 0.6, seed=0, connect=False))
+    assert not np.array_equal(g, prtlb.disk_pack((128, 128), 6.0, 0.6, seed=1, connect=False))
+    s = prtlb.sphere_pack((48, 40, 32), (4.0, 8.0), 0.35, seed=3, connect=False)
+    phi3 = float((s == prtlb.PORE).mean())
+    assert 0.28 < phi3 <= 0.35
+    # periodic minimum image: the pack tiles, so a grain crossing x = 0
+    # shows on both faces somewhere
+    assert (s[0] == 0).any() and (s[-1] == 0).any()
+
+
+KEYS = {"nan": ["not finite"], "good": [], "over_cap": ["above"],
+        "negative": ["negative"], "zero": ["exactly zero"], "wrap": ["wrapping"]}
+
+
+@pytest.mark.parametrize("case", sorted(KEYS))
+def test_check_physics_matches_reference(case):
+    d = golden("check_physics")
+    ref = d["meta"]["complaints"][case]
+    got = prtlb.check_physics(d["out_" + case], d["mat_" + case])
+    assert len(got) == len(ref)
+    for kw in KEYS[case]:
+        assert any(kw in s for s in got)
+
+
+def test_host_errors():
+    with pytest.raises(ValueError):
+        prtlb.trt_minus(1.0 / 0.5)
+    g = np.full((10, 4), 2, np.uint8)
+    v = np.zeros((3, 10, 4), np.float32)
+    with pytest.raises(ValueError):
+        adr_host_setup(g, v, 10.0, 0.0, 2, da_abio=1.0)      # abiotic index 2, 2 chemicals
+    with pytest.raises(ValueError):
+        adr_host_setup(np.zeros((4,), np.uint8), np.zeros((3, 4), np.float32), 1.0, 0.0, 1)
+    with pytest.raises(ValueError):
+        adr_host_setup(g, v, 10.0, 0.0, 3, chem=prtlb.default_chemistry(2))
+
+
+def test_device_required():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    with pytest.raises(prtlb.PrtError):
+        prtlb.stokes_d2q9(np.full((8, 6), 2, np.uint8), nit=2)
