@@ -1,0 +1,179 @@
+"""Benchmark of the PRT-LB hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one fused D3Q19 TRT iteration (pull-stream + collide + Guo force +
+solid reversal) over the whole 128^3 sphere pack (radii U(4,8), porosity
+~0.35, seed 0), FP64 populations: the D3Q19 Stokes stage of the plain GPU
+drop-in workload.  Populations, mask and partial-sum buffers are resident in
+HBM before the timed region; K steps are timed with HIP events on the stream
+the kernels run on.  `value` is box-node MLUPS; the roofline object prices
+the 304 B/node algorithmic traffic against 8 TB/s.  Extra keys report the
+FP32 / FP16-shifted storages, the 256^3 grid, the fused transport step and
+the CPU baseline (this repo's NumPy oracle on the host, one core).
+Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "prt-deeponet-gui_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+PEAK_HBM = 8.0e12          # MI355X HBM3E, bytes/s
+BYTES_PER_LU = {("fp64", 19): 304, ("fp32", 19): 152, ("fp16", 19): 76,
+                ("fp64", 9): 144, ("fp32", 9): 72, ("fp16", 9): 36}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_lbm(g, storage, steps, warmup, skip_solid=False):
+    import torch
+    import prtlb
+    s = prtlb.StokesSolver(g, force=2e-6, storage=storage, skip_solid=skip_solid)
+    s.run(warmup)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    s.run(steps)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    nodes = s.n_fluid if skip_solid else s.n
+    mlups = nodes / (ms * 1e-3) / 1e6
+    bpl = BYTES_PER_LU[(storage, s.q)]
+    gbs = nodes * bpl / (ms * 1e-3) / 1e9
+    out = dict(ms_per_step=ms, mlups=mlups, gbs=gbs, frac=gbs * 1e9 / PEAK_HBM,
+               nodes=nodes, fluid_fraction=s.n_fluid / s.n, bytes_per_lu=bpl,
+               mlups_fluid=s.n_fluid / (ms * 1e-3) / 1e6)
+    del s
+    torch.cuda.empty_cache()
+    return out
+
+
+def time_adr(g, steps, warmup):
+    import numpy as np
+    import torch
+    import prtlb
+    v = prtlb.stokes_d3q19(g, nit=1500, force=2e-6, tol=1e-5, storage="fp32")
+    T = prtlb.TransportSolver(g, v, 10.0, 1.0, 1, chem=prtlb.chemistry.monod(0.5))
+    T.step_n(warmup)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    T.step_n(steps)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    vps = T.n_pore / (ms * 1e-3)
+    bpv = 2 * 1 * 4 + 3 * 4 + 1          # 21 B per pore voxel-step (ns = 1, 3D, f32)
+    return dict(ms_per_step=ms, pore_voxel_steps_per_s=vps, gbs=vps * bpv / 1e9,
+                frac=vps * bpv / PEAK_HBM, bytes_per_voxel=bpv, pore_voxels=T.n_pore)
+
+
+def cpu_baseline(budget_s=15.0):
+    """The NumPy oracle (a restatement of the reference's vectorised NumPy
+    loop) timed on the host: D3Q19 FP64 on a 64^3 sphere pack, one core."""
+    import numpy as np
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    import prtlb
+    from oracle import prtlb_oracle as O
+    g = prtlb.sphere_pack((64, 64, 64), (4.0, 8.0), 0.35, seed=0)
+    t0 = time.perf_counter()
+    O.stokes(g, nit=2, force=2e-6, tol=0.0)
+    per = (time.perf_counter() - t0) / 2
+    nit = max(2, int(budget_s / max(per, 1e-3)))
+    t0 = time.perf_counter()
+    O.stokes(g, nit=nit, force=2e-6, tol=0.0)
+    dt = time.perf_counter() - t0
+    return dict(value=g.size * nit / dt / 1e6, unit="MLUPS", cores=1, kind="port",
+                sample="NumPy oracle D3Q19 FP64 TRT, 64^3 sphere pack, %d iterations "
+                       "(%.1f s) on %s" % (nit, dt, platform.processor() or platform.machine()))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--grid", type=int, default=128, help="edge of the main cube")
+    ap.add_argument("--storage", default="fp64", choices=["fp64", "fp32", "fp16"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import prtlb
+    assert torch.cuda.is_available(), "bench needs a HIP device"
+
+    E = args.grid
+    log("geometry %d^3 ..." % E)
+    gmain = prtlb.sphere_pack((E, E, E), (4.0, 8.0), 0.35, seed=0)
+    log("%s %d^3 ..." % (args.storage, E))
+    main_r = time_lbm(gmain, args.storage, args.steps, args.warmup)
+    g128 = gmain if E == 128 else None
+    extras = {}
+    if not args.no_extras:
+        if g128 is None:
+            g128 = prtlb.sphere_pack((128, 128, 128), (4.0, 8.0), 0.35, seed=0)
+        for st in ("fp32", "fp16"):
+            log(st, "128^3 ...")
+            extras["d3q19_128_" + st] = time_lbm(g128, st, args.steps, args.warmup)
+        log("fluid-only fp64 128^3 ...")
+        extras["d3q19_128_fp64_fluid_only"] = time_lbm(g128, "fp64", args.steps, args.warmup,
+                                                       skip_solid=True)
+        log("geometry 256^3 ...")
+        g256 = prtlb.sphere_pack((256, 256, 256), (4.0, 8.0), 0.35, seed=0)
+        for st in ("fp64", "fp32", "fp16"):
+            log(st, "256^3 ...")
+            extras["d3q19_256_" + st] = time_lbm(g256, st, max(args.steps // 2, 20), args.warmup)
+        del g256
+        log("transport 128^3 ...")
+        extras["adr_128_monod_f32"] = time_adr(g128, args.steps, args.warmup)
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline()
+    line = {
+        "metric": "D3Q19 TRT MLUPS & %HBM peak; batched solves/hr + p99 latency at fixed error",
+        "value": round(main_r["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_r["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f16 storage, f32 arithmetic"}[args.storage],
+        "data": "synthetic: seeded %d^3 overlapping-sphere pack, radii U(4,8), porosity %.3f"
+                % (E, main_r["fluid_fraction"]),
+        "config": {"workload": "D3Q19 TRT Stokes step, %d^3 sphere pack, %s populations, "
+                               "dense box (every node updated)" % (E, args.storage.upper()),
+                   "grid": [E, E, E], "lattice": "D3Q19", "storage": args.storage,
+                   "parallelism": "none (1 GPU)"},
+        "roofline": {"bound": "hbm", "achieved": round(main_r["gbs"], 1),
+                     "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": round(main_r["frac"], 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "mlups_fluid": round(main_r["mlups_fluid"], 1),
+        "extras": extras,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -96,12 +96,16 @@ int prt_sum_f64(const double *x, int64_t n, double *out, void *stream);
 
 /* Velocity from the post-collision state f_last (pull, density, Guo-shifted
  * momentum, zero in solid).  Replaces 3D/tools/prtlb_3d.py:451-455 and
- * 3D/tools/prtlb_2d.py:481-485.  v64 (3, n) double and/or v32 (3, n) float,
- * either may be NULL; for 2D the third row is 0.  partials (device, may be
- * NULL, prt_lbm_partials_count() long) receives per-block FP64 sums of u_x
- * over every node, for the Darcy permeability (reduce with prt_sum_f64). */
+ * 3D/tools/prtlb_2d.py:481-485.  With PRT_LBM_SKIP_SOLID in flags, f_prev is
+ * the other buffer of the pair (the state one iteration older), as
+ * prt_lbm_step's fluid-only mode requires; otherwise it may be NULL.
+ * v64 (3, n) double and/or v32 (3, n) float, either may be NULL; for 2D the
+ * third row is 0.  partials (device, may be NULL, prt_lbm_partials_count()
+ * long) receives per-block FP64 sums of u_x over every node, for the Darcy
+ * permeability (reduce with prt_sum_f64). */
 int prt_lbm_velocity(int q, int storage, int nx, int ny, int nz,
-                     const uint8_t *solid, const void *f_last, double force,
+                     const uint8_t *solid, const void *f_last,
+                     const void *f_prev, int flags, double force,
                      double fp16_scale, double *v64, float *v32,
                      double *partials, void *stream);
 
@@ -168,6 +172,64 @@ int prt_adr_step_f64(const prt_adr_params *p, int nx, int ny, int nz,
 int prt_adr_run(const prt_adr_params *p, int nx, int ny, int nz,
                 const uint8_t *flags, const float *u, float *c_a, float *c_b,
                 int steps, void *stream);
+
+/* ------------------------------------------------------------- batching */
+
+/* Many geometries of uneven size advanced by ONE launch (the batched
+ * campaign and the serving loop).  No counterpart in the reference, which
+ * solves one geometry per process; each segment is one call of
+ * stokes_d2q9/stokes_d3q19 (or one solve_adr step) and gives bit-identical
+ * results alone or in any batch. */
+typedef struct prt_lbm_segment {
+  const uint8_t *solid;   /* n bytes, device */
+  void *f_a, *f_b;        /* the segment's two population buffers (q * n) */
+  int32_t nx, ny, nz;     /* nz = 1 for D2Q9 */
+  int32_t phase;          /* which buffer is newest at launch parity 0:
+                             0 = f_a, 1 = f_b */
+  double omega_p, omega_m, force, fp16_scale;
+} prt_lbm_segment;
+
+typedef struct prt_adr_segment {
+  const prt_adr_params *params;   /* host pointer, read by the plan call */
+  int32_t nx, ny, nz, phase;
+  const uint8_t *flags;           /* device, as prt_adr_step */
+  const float *u;
+  float *c_a, *c_b;
+} prt_adr_segment;
+
+/* Bytes of one device descriptor of each kind. */
+int64_t prt_lbm_batch_desc_size(void);
+int64_t prt_adr_batch_desc_size(void);
+
+/* Build the descriptors (desc_host: nseg * desc_size bytes, HOST) and the
+ * block-to-segment table (blk_seg_host, HOST, NULL = size query into
+ * *n_blocks); the caller copies both to the device. */
+int prt_lbm_batch_plan(int q, const prt_lbm_segment *segs, int nseg,
+                       void *desc_host, int32_t *blk_seg_host,
+                       int64_t blk_capacity, int64_t *n_blocks);
+int prt_adr_batch_plan(const prt_adr_segment *segs, int nseg, void *desc_host,
+                       int32_t *blk_seg_host, int64_t blk_capacity,
+                       int64_t *n_blocks, int *max_species);
+
+/* One iteration of every segment.  parity flips each launch; partials
+ * (device, n_blocks doubles; NULL = none) receive per-block sums of |u_x|
+ * over fluid nodes. */
+int prt_lbm_batch_step(int q, int storage, const void *desc_dev,
+                       const int32_t *blk_seg_dev, int64_t n_blocks,
+                       int parity, int flags, double *partials, void *stream);
+
+/* One transport step of every segment; partials (NULL = none) receive per
+ * block, per chemical slot, FP64 pore sums of the new field: width is 1, 2,
+ * 4 or 8 (the smallest holding max_species). */
+int prt_adr_batch_step(const void *desc_dev, const int32_t *blk_seg_dev,
+                       int64_t n_blocks, int max_species, int parity,
+                       double *partials, void *stream);
+
+/* out[s * width + j] = sum of partials[b * width + j] over the blocks
+ * [blk0[s], blk0[s] + nblk[s]) of segment s, in a fixed order. */
+int prt_seg_sums(const int32_t *blk0_dev, const int32_t *nblk_dev, int nseg,
+                 int width, const double *partials, double *out,
+                 void *stream);
 
 /* ------------------------------------------------------ NumPy-order sums */
 

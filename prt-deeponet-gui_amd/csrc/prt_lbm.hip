@@ -22,214 +22,9 @@
 #include <cmath>
 #include <cstring>
 
-#include "prt_common.h"
+#include "prt_lbm_core.h"
 
 namespace prt {
-
-template <int Q>
-struct Lat;
-
-// D3Q19, ordering of 3D/tools/prtlb_3d.py:370-375 (required for population
-// level parity with the reference's state).
-template <>
-struct Lat<19> {
-  static constexpr int cx[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
-  static constexpr int cy[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
-  static constexpr int cz[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
-  static constexpr int opp[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
-  static constexpr int cls[19] = {0, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};
-};
-// D2Q9 of 3D/tools/prtlb_2d.py:391-394, stored as (x, -, y) -> (x, y=0, z).
-template <>
-struct Lat<9> {
-  static constexpr int cx[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
-  static constexpr int cy[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  static constexpr int cz[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
-  static constexpr int opp[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
-  static constexpr int cls[9] = {0, 1, 1, 1, 1, 2, 2, 2, 2};
-};
-
-template <int Q>
-__host__ __device__ constexpr double weight(int i) {
-  return Q == 19 ? (Lat<19>::cls[i] == 0 ? 1.0 / 3.0 : Lat<19>::cls[i] == 1 ? 1.0 / 18.0 : 1.0 / 36.0)
-                 : (Lat<9>::cls[i] == 0 ? 4.0 / 9.0 : Lat<9>::cls[i] == 1 ? 1.0 / 9.0 : 1.0 / 36.0);
-}
-
-struct LbmArgs {
-  const uint8_t *solid;
-  const void *src;
-  void *dst;
-  double *partials;
-  int X, Y, Z;
-  long long N;
-  double omp, omm, g, half_g;
-  double guo[19];     // w_i (1 - omm/2) 3 g e_ix, formed as the reference does
-  float guo32[19];
-  float ld16[19];     // w_i / s     (FP16 load scale)
-  float st16[19];     // s / w_i     (FP16 store scale)
-};
-
-// ---------------------------------------------------------------- storage
-template <int ST>
-struct Pop;
-template <>
-struct Pop<PRT_STORE_FP64> {
-  using T = double;
-};
-template <>
-struct Pop<PRT_STORE_FP32> {
-  using T = float;
-};
-template <>
-struct Pop<PRT_STORE_FP16> {
-  using T = __half;
-};
-
-// Gather index of the node that streams into (x, y, z) along e_i.
-struct Nbr {
-  int xm, xp, ym, yp, zm, zp, x, y, z, Y, Z;
-  __device__ inline int at(int ex, int ey, int ez) const {
-    const int sx = ex > 0 ? xm : (ex < 0 ? xp : x);
-    const int sy = ey > 0 ? ym : (ey < 0 ? yp : y);
-    const int sz = ez > 0 ? zm : (ez < 0 ? zp : z);
-    return (sx * Y + sy) * Z + sz;
-  }
-};
-
-__device__ inline Nbr make_nbr(int x, int y, int z, int X, int Y, int Z) {
-  Nbr b;
-  b.x = x; b.y = y; b.z = z; b.Y = Y; b.Z = Z;
-  b.xm = x == 0 ? X - 1 : x - 1;
-  b.xp = x == X - 1 ? 0 : x + 1;
-  b.ym = y == 0 ? Y - 1 : y - 1;
-  b.yp = y == Y - 1 ? 0 : y + 1;
-  b.zm = z == 0 ? Z - 1 : z - 1;
-  b.zp = z == Z - 1 ? 0 : z + 1;
-  return b;
-}
-
-// Load the Q populations streaming into node n (SKIP: from the node's own
-// two-step-old opposite slot where the upstream node is solid).
-template <int Q, int ST, bool SKIP, typename R>
-__device__ inline void gather(const LbmArgs &a, const Nbr &nb, int n, R *f) {
-  using T = typename Pop<ST>::T;
-  const T *src = static_cast<const T *>(a.src);
-  const T *own = static_cast<const T *>(a.dst);
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const int s = (i == 0) ? n : nb.at(Lat<Q>::cx[i], Lat<Q>::cy[i], Lat<Q>::cz[i]);
-    T v;
-    if (SKIP && i != 0 && a.solid[s])
-      v = own[(size_t)Lat<Q>::opp[i] * a.N + n];
-    else
-      v = src[(size_t)i * a.N + s];
-    if constexpr (ST == PRT_STORE_FP16)
-      f[i] = __half2float(v) * a.ld16[i];
-    else
-      f[i] = (R)v;
-  }
-}
-
-template <int Q, int ST, typename R>
-__device__ inline void scatter(const LbmArgs &a, int n, const R *fp) {
-  using T = typename Pop<ST>::T;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    if constexpr (ST == PRT_STORE_FP16) {
-      unsigned short *dst = static_cast<unsigned short *>(a.dst);
-      const unsigned short v = __half_as_ushort(__float2half_rn(fp[i] * a.st16[i]));
-      __builtin_nontemporal_store(v, dst + (size_t)i * a.N + n);
-    } else {
-      T *dst = static_cast<T *>(a.dst);
-      __builtin_nontemporal_store((T)fp[i], dst + (size_t)i * a.N + n);
-    }
-  }
-}
-
-// FP64 collision in the reference's own evaluation order (contraction off):
-// rho = sum f, j = einsum(e, f), u = (j + g/2 e_x)/rho masked, feq, TRT with
-// f+/- = (f +/- f_opp)/2, then + Guo.  3D/tools/prtlb_3d.py:142-154, 427-440.
-template <int Q>
-__device__ inline void collide_ref(const LbmArgs &a, bool solid, const double *f,
-                                   double *fp, double &ux_out) {
-#pragma clang fp contract(off)
-  double rho = f[0];
-#pragma unroll
-  for (int i = 1; i < Q; ++i) rho += f[i];
-  double jx = 0.0, jy = 0.0, jz = 0.0;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    if (Lat<Q>::cx[i]) jx += (double)Lat<Q>::cx[i] * f[i];
-    if (Lat<Q>::cy[i]) jy += (double)Lat<Q>::cy[i] * f[i];
-    if (Lat<Q>::cz[i]) jz += (double)Lat<Q>::cz[i] * f[i];
-  }
-  double ux = (jx + a.half_g) / rho;
-  double uy = jy / rho;
-  double uz = jz / rho;
-  if (solid) ux = uy = uz = 0.0;
-  ux_out = ux;
-  const double usq = (ux * ux + uy * uy) + uz * uz;
-  double feq[Q];
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const double eu = ((double)Lat<Q>::cx[i] * ux + (double)Lat<Q>::cy[i] * uy) +
-                      (double)Lat<Q>::cz[i] * uz;
-    feq[i] = (weight<Q>(i) * rho) * (((1.0 + 3.0 * eu) + (4.5 * eu) * eu) - 1.5 * usq);
-  }
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const int o = Lat<Q>::opp[i];
-    const double f_p = 0.5 * (f[i] + f[o]);
-    const double f_m = 0.5 * (f[i] - f[o]);
-    const double e_p = 0.5 * (feq[i] + feq[o]);
-    const double e_m = 0.5 * (feq[i] - feq[o]);
-    double v = (f[i] - a.omp * (f_p - e_p)) - a.omm * (f_m - e_m);
-    v = v + a.guo[i];
-    fp[i] = solid ? f[o] : v;
-  }
-}
-
-// FP32 collision on deviations g_i = f_i - w_i (the shifted storages).
-template <int Q>
-__device__ inline void collide_dev(const LbmArgs &a, bool solid, const float *gq,
-                                   float *out, double &ux_out) {
-  float dr = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    dr += gq[i];
-    if (Lat<Q>::cx[i]) jx += Lat<Q>::cx[i] > 0 ? gq[i] : -gq[i];
-    if (Lat<Q>::cy[i]) jy += Lat<Q>::cy[i] > 0 ? gq[i] : -gq[i];
-    if (Lat<Q>::cz[i]) jz += Lat<Q>::cz[i] > 0 ? gq[i] : -gq[i];
-  }
-  const float rho = 1.f + dr;
-  const float irho = 1.f / rho;
-  float ux = (jx + (float)a.half_g) * irho, uy = jy * irho, uz = jz * irho;
-  if (solid) ux = uy = uz = 0.f;
-  ux_out = ux;
-  const float usq = ux * ux + uy * uy + uz * uz;
-  const float omp = (float)a.omp, omm = (float)a.omm;
-  {
-    const float eq0 = (float)weight<Q>(0) * (dr - 1.5f * rho * usq);
-    out[0] = solid ? gq[0] : gq[0] - omp * (gq[0] - eq0);
-  }
-#pragma unroll
-  for (int i = 1; i < Q; ++i) {
-    const int o = Lat<Q>::opp[i];
-    if (o < i) continue;
-    const float w = (float)weight<Q>(i);
-    const float eu = (float)Lat<Q>::cx[i] * ux + (float)Lat<Q>::cy[i] * uy + (float)Lat<Q>::cz[i] * uz;
-    const float sym = w * (dr + rho * (4.5f * eu * eu - 1.5f * usq));
-    const float asym = w * rho * 3.f * eu;
-    const float gp = 0.5f * (gq[i] + gq[o]);
-    const float gm = 0.5f * (gq[i] - gq[o]);
-    const float A = omp * (gp - sym);
-    const float B = omm * (gm - asym);
-    const float vi = gq[i] - A - B + a.guo32[i];
-    const float vo = gq[o] - A + B + a.guo32[o];
-    out[i] = solid ? gq[o] : vi;
-    out[o] = solid ? gq[i] : vo;
-  }
-}
 
 template <int Q, int ST, bool SKIP, bool RED>
 __global__ __launch_bounds__(256) void lbm_step_kernel(LbmArgs a) {
@@ -267,7 +62,7 @@ __global__ __launch_bounds__(256) void lbm_step_kernel(LbmArgs a) {
 }
 
 // Macroscopic field of the final state (3D/tools/prtlb_3d.py:451-455).
-template <int Q, int ST>
+template <int Q, int ST, bool SKIP>
 __global__ __launch_bounds__(256) void lbm_velocity_kernel(LbmArgs a, double *v64, float *v32) {
 #pragma clang fp contract(off)
   __shared__ double lds[16];
@@ -282,7 +77,7 @@ __global__ __launch_bounds__(256) void lbm_velocity_kernel(LbmArgs a, double *v6
     double u[3];
     if constexpr (ST == PRT_STORE_FP64) {
       double f[Q];
-      gather<Q, ST, false>(a, nb, n, f);
+      gather<Q, ST, SKIP>(a, nb, n, f);
       double rho = f[0];
 #pragma unroll
       for (int i = 1; i < Q; ++i) rho += f[i];
@@ -298,7 +93,7 @@ __global__ __launch_bounds__(256) void lbm_velocity_kernel(LbmArgs a, double *v6
       u[2] = j[2] / rho;
     } else {
       float f[Q];
-      gather<Q, ST, false>(a, nb, n, f);
+      gather<Q, ST, SKIP>(a, nb, n, f);
       double dr = 0.0, j[3] = {0.0, 0.0, 0.0};
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
@@ -421,18 +216,9 @@ static void fill_args(LbmArgs &a, int q, const Geom &g, const uint8_t *solid, co
   a.solid = solid; a.src = src; a.dst = dst; a.partials = partials;
   a.X = g.X; a.Y = g.Y; a.Z = g.Z; a.N = g.N;
   a.omp = omp; a.omm = omm; a.g = force; a.half_g = 0.5 * force;
-  for (int i = 0; i < q; ++i) {
-    const double w = q == 19 ? weight<19>(i) : weight<9>(i);
-    const int ex = q == 19 ? Lat<19>::cx[i] : Lat<9>::cx[i];
-    // same association as the reference: w * (1 - 0.5 om_m) * 3.0 * gf * ex
-    volatile double t = w * (1 - 0.5 * omm);
-    t = t * 3.0;
-    t = t * force;
-    a.guo[i] = t * (double)ex;
-    a.guo32[i] = (float)a.guo[i];
-    a.ld16[i] = (float)(w / s16);
-    a.st16[i] = (float)(s16 / w);
-  }
+  a.omh = 1 - 0.5 * omm;
+  a.s16 = (float)s16;
+  a.inv_s16 = (float)(1.0 / s16);
 }
 
 template <int Q, int ST>
@@ -556,29 +342,40 @@ extern "C" int prt_sum_f64(const double *x, int64_t n, double *out, void *stream
 }
 
 extern "C" int prt_lbm_velocity(int q, int storage, int nx, int ny, int nz, const uint8_t *solid,
-                                const void *f_last, double force, double fp16_scale, double *v64, float *v32,
-                                double *partials, void *stream) {
+                                const void *f_last, const void *f_prev, int flags, double force,
+                                double fp16_scale, double *v64, float *v32, double *partials, void *stream) {
   Geom g;
   int rc = make_geom(q, nx, ny, nz, g);
   if (rc) return rc;
-  if (!solid || !f_last || (storage == PRT_STORE_FP16 && !(fp16_scale > 0))) {
+  const bool skip = (flags & PRT_LBM_SKIP_SOLID) != 0;
+  if (!solid || !f_last || (skip && (!f_prev || f_prev == f_last)) ||
+      (storage == PRT_STORE_FP16 && !(fp16_scale > 0))) {
     set_error("prt_lbm_velocity: bad arguments");
     return PRT_E_ARG;
   }
   LbmArgs a;
-  fill_args(a, q, g, solid, f_last, nullptr, 1.0, 1.0, force, fp16_scale > 0 ? fp16_scale : 1.0, partials);
+  // in the fluid-only mode the populations a solid neighbour would hand
+  // back are a fluid node's own two-step-old slots: the other buffer
+  fill_args(a, q, g, solid, f_last, const_cast<void *>(f_prev), 1.0, 1.0, force,
+            fp16_scale > 0 ? fp16_scale : 1.0, partials);
   hipStream_t st = (hipStream_t)stream;
+#define PRT_VK(QQ, SS)                                                                              \
+  do {                                                                                              \
+    if (skip) lbm_velocity_kernel<QQ, SS, true><<<g.grid, g.block, 0, st>>>(a, v64, v32);            \
+    else lbm_velocity_kernel<QQ, SS, false><<<g.grid, g.block, 0, st>>>(a, v64, v32);                \
+  } while (0)
   switch (q * 4 + storage) {
-    case 19 * 4 + PRT_STORE_FP64: lbm_velocity_kernel<19, PRT_STORE_FP64><<<g.grid, g.block, 0, st>>>(a, v64, v32); break;
-    case 19 * 4 + PRT_STORE_FP32: lbm_velocity_kernel<19, PRT_STORE_FP32><<<g.grid, g.block, 0, st>>>(a, v64, v32); break;
-    case 19 * 4 + PRT_STORE_FP16: lbm_velocity_kernel<19, PRT_STORE_FP16><<<g.grid, g.block, 0, st>>>(a, v64, v32); break;
-    case 9 * 4 + PRT_STORE_FP64: lbm_velocity_kernel<9, PRT_STORE_FP64><<<g.grid, g.block, 0, st>>>(a, v64, v32); break;
-    case 9 * 4 + PRT_STORE_FP32: lbm_velocity_kernel<9, PRT_STORE_FP32><<<g.grid, g.block, 0, st>>>(a, v64, v32); break;
-    case 9 * 4 + PRT_STORE_FP16: lbm_velocity_kernel<9, PRT_STORE_FP16><<<g.grid, g.block, 0, st>>>(a, v64, v32); break;
+    case 19 * 4 + PRT_STORE_FP64: PRT_VK(19, PRT_STORE_FP64); break;
+    case 19 * 4 + PRT_STORE_FP32: PRT_VK(19, PRT_STORE_FP32); break;
+    case 19 * 4 + PRT_STORE_FP16: PRT_VK(19, PRT_STORE_FP16); break;
+    case 9 * 4 + PRT_STORE_FP64: PRT_VK(9, PRT_STORE_FP64); break;
+    case 9 * 4 + PRT_STORE_FP32: PRT_VK(9, PRT_STORE_FP32); break;
+    case 9 * 4 + PRT_STORE_FP16: PRT_VK(9, PRT_STORE_FP16); break;
     default:
       set_error("unknown storage %d", storage);
       return PRT_E_ARG;
   }
+#undef PRT_VK
   return launch_status("prt_lbm_velocity launch");
 }
 

@@ -1,0 +1,240 @@
+// prt_lbm_core.h -- device building blocks of the TRT lattice-Boltzmann
+// kernels: lattice tables, population storage, the periodic pull gather,
+// the post-collision scatter and the two collision forms.  Shared by the
+// single-geometry kernels (prt_lbm.hip) and the batched multi-geometry
+// kernels (prt_batch.hip) so both evaluate bit-identical arithmetic.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "prt_common.h"
+
+namespace prt {
+
+template <int Q>
+struct Lat;
+
+// D3Q19, ordering of 3D/tools/prtlb_3d.py:370-375 (required for population
+// level parity with the reference's state).
+template <>
+struct Lat<19> {
+  static constexpr int cx[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+  static constexpr int cy[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+  static constexpr int cz[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+  static constexpr int opp[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+  static constexpr int cls[19] = {0, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};
+};
+// D2Q9 of 3D/tools/prtlb_2d.py:391-394, stored as (x, -, y) -> (x, y=0, z).
+template <>
+struct Lat<9> {
+  static constexpr int cx[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+  static constexpr int cy[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  static constexpr int cz[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+  static constexpr int opp[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+  static constexpr int cls[9] = {0, 1, 1, 1, 1, 2, 2, 2, 2};
+};
+
+template <int Q>
+__host__ __device__ constexpr double weight(int i) {
+  return Q == 19 ? (Lat<19>::cls[i] == 0 ? 1.0 / 3.0 : Lat<19>::cls[i] == 1 ? 1.0 / 18.0 : 1.0 / 36.0)
+                 : (Lat<9>::cls[i] == 0 ? 4.0 / 9.0 : Lat<9>::cls[i] == 1 ? 1.0 / 9.0 : 1.0 / 36.0);
+}
+
+struct LbmArgs {
+  const uint8_t *solid;
+  const void *src;
+  void *dst;
+  double *partials;
+  int X, Y, Z;
+  long long N;
+  double omp, omm, g, half_g;
+  double omh;         // 1 - omm/2, formed on the host as the reference does
+  float s16, inv_s16; // FP16 storage scale s and 1/s
+};
+
+// Guo forcing term w_i (1 - omm/2) 3 g e_ix with the reference's association
+// (3D/tools/prtlb_3d.py:439-440); exact constants, contraction off.
+template <int Q>
+__device__ inline double guo_term(const LbmArgs &a, int i) {
+#pragma clang fp contract(off)
+  double t = weight<Q>(i) * a.omh;
+  t = t * 3.0;
+  t = t * a.g;
+  return t * (double)Lat<Q>::cx[i];
+}
+
+// FP16 storage: h_i = (f_i - w_i) * s / w_i; 1/w_i is an exact constant.
+template <int Q>
+__device__ inline float ld16(const LbmArgs &a, int i) {
+  return (float)weight<Q>(i) * a.inv_s16;
+}
+template <int Q>
+__device__ inline float st16(const LbmArgs &a, int i) {
+  return a.s16 * (float)(1.0 / weight<Q>(i));
+}
+
+// ---------------------------------------------------------------- storage
+template <int ST>
+struct Pop;
+template <>
+struct Pop<PRT_STORE_FP64> {
+  using T = double;
+};
+template <>
+struct Pop<PRT_STORE_FP32> {
+  using T = float;
+};
+template <>
+struct Pop<PRT_STORE_FP16> {
+  using T = __half;
+};
+
+// Gather index of the node that streams into (x, y, z) along e_i.
+struct Nbr {
+  int xm, xp, ym, yp, zm, zp, x, y, z, Y, Z;
+  __device__ inline int at(int ex, int ey, int ez) const {
+    const int sx = ex > 0 ? xm : (ex < 0 ? xp : x);
+    const int sy = ey > 0 ? ym : (ey < 0 ? yp : y);
+    const int sz = ez > 0 ? zm : (ez < 0 ? zp : z);
+    return (sx * Y + sy) * Z + sz;
+  }
+};
+
+__device__ inline Nbr make_nbr(int x, int y, int z, int X, int Y, int Z) {
+  Nbr b;
+  b.x = x; b.y = y; b.z = z; b.Y = Y; b.Z = Z;
+  b.xm = x == 0 ? X - 1 : x - 1;
+  b.xp = x == X - 1 ? 0 : x + 1;
+  b.ym = y == 0 ? Y - 1 : y - 1;
+  b.yp = y == Y - 1 ? 0 : y + 1;
+  b.zm = z == 0 ? Z - 1 : z - 1;
+  b.zp = z == Z - 1 ? 0 : z + 1;
+  return b;
+}
+
+// Load the Q populations streaming into node n (SKIP: from the node's own
+// two-step-old opposite slot where the upstream node is solid).
+template <int Q, int ST, bool SKIP, typename R>
+__device__ inline void gather(const LbmArgs &a, const Nbr &nb, int n, R *f) {
+  using T = typename Pop<ST>::T;
+  const T *src = static_cast<const T *>(a.src);
+  const T *own = static_cast<const T *>(a.dst);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int s = (i == 0) ? n : nb.at(Lat<Q>::cx[i], Lat<Q>::cy[i], Lat<Q>::cz[i]);
+    T v;
+    if (SKIP && i != 0 && a.solid[s])
+      v = own[(size_t)Lat<Q>::opp[i] * a.N + n];
+    else
+      v = src[(size_t)i * a.N + s];
+    if constexpr (ST == PRT_STORE_FP16)
+      f[i] = __half2float(v) * ld16<Q>(a, i);
+    else
+      f[i] = (R)v;
+  }
+}
+
+template <int Q, int ST, typename R>
+__device__ inline void scatter(const LbmArgs &a, int n, const R *fp) {
+  using T = typename Pop<ST>::T;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    if constexpr (ST == PRT_STORE_FP16) {
+      unsigned short *dst = static_cast<unsigned short *>(a.dst);
+      const unsigned short v = __half_as_ushort(__float2half_rn(fp[i] * st16<Q>(a, i)));
+      __builtin_nontemporal_store(v, dst + (size_t)i * a.N + n);
+    } else {
+      T *dst = static_cast<T *>(a.dst);
+      __builtin_nontemporal_store((T)fp[i], dst + (size_t)i * a.N + n);
+    }
+  }
+}
+
+// FP64 collision in the reference's own evaluation order (contraction off):
+// rho = sum f, j = einsum(e, f), u = (j + g/2 e_x)/rho masked, feq, TRT with
+// f+/- = (f +/- f_opp)/2, then + Guo.  3D/tools/prtlb_3d.py:142-154, 427-440.
+template <int Q>
+__device__ inline void collide_ref(const LbmArgs &a, bool solid, const double *f,
+                                   double *fp, double &ux_out) {
+#pragma clang fp contract(off)
+  double rho = f[0];
+#pragma unroll
+  for (int i = 1; i < Q; ++i) rho += f[i];
+  double jx = 0.0, jy = 0.0, jz = 0.0;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    if (Lat<Q>::cx[i]) jx += (double)Lat<Q>::cx[i] * f[i];
+    if (Lat<Q>::cy[i]) jy += (double)Lat<Q>::cy[i] * f[i];
+    if (Lat<Q>::cz[i]) jz += (double)Lat<Q>::cz[i] * f[i];
+  }
+  double ux = (jx + a.half_g) / rho;
+  double uy = jy / rho;
+  double uz = jz / rho;
+  if (solid) ux = uy = uz = 0.0;
+  ux_out = ux;
+  const double usq = (ux * ux + uy * uy) + uz * uz;
+  double feq[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const double eu = ((double)Lat<Q>::cx[i] * ux + (double)Lat<Q>::cy[i] * uy) +
+                      (double)Lat<Q>::cz[i] * uz;
+    feq[i] = (weight<Q>(i) * rho) * (((1.0 + 3.0 * eu) + (4.5 * eu) * eu) - 1.5 * usq);
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int o = Lat<Q>::opp[i];
+    const double f_p = 0.5 * (f[i] + f[o]);
+    const double f_m = 0.5 * (f[i] - f[o]);
+    const double e_p = 0.5 * (feq[i] + feq[o]);
+    const double e_m = 0.5 * (feq[i] - feq[o]);
+    double v = (f[i] - a.omp * (f_p - e_p)) - a.omm * (f_m - e_m);
+    v = v + guo_term<Q>(a, i);
+    fp[i] = solid ? f[o] : v;
+  }
+}
+
+// FP32 collision on deviations g_i = f_i - w_i (the shifted storages).
+template <int Q>
+__device__ inline void collide_dev(const LbmArgs &a, bool solid, const float *gq,
+                                   float *out, double &ux_out) {
+  float dr = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    dr += gq[i];
+    if (Lat<Q>::cx[i]) jx += Lat<Q>::cx[i] > 0 ? gq[i] : -gq[i];
+    if (Lat<Q>::cy[i]) jy += Lat<Q>::cy[i] > 0 ? gq[i] : -gq[i];
+    if (Lat<Q>::cz[i]) jz += Lat<Q>::cz[i] > 0 ? gq[i] : -gq[i];
+  }
+  const float rho = 1.f + dr;
+  const float irho = 1.f / rho;
+  float ux = (jx + (float)a.half_g) * irho, uy = jy * irho, uz = jz * irho;
+  if (solid) ux = uy = uz = 0.f;
+  ux_out = ux;
+  const float usq = ux * ux + uy * uy + uz * uz;
+  const float omp = (float)a.omp, omm = (float)a.omm;
+  {
+    const float eq0 = (float)weight<Q>(0) * (dr - 1.5f * rho * usq);
+    out[0] = solid ? gq[0] : gq[0] - omp * (gq[0] - eq0);
+  }
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    const int o = Lat<Q>::opp[i];
+    if (o < i) continue;
+    const float w = (float)weight<Q>(i);
+    const float eu = (float)Lat<Q>::cx[i] * ux + (float)Lat<Q>::cy[i] * uy + (float)Lat<Q>::cz[i] * uz;
+    const float sym = w * (dr + rho * (4.5f * eu * eu - 1.5f * usq));
+    const float asym = w * rho * 3.f * eu;
+    const float gp = 0.5f * (gq[i] + gq[o]);
+    const float gm = 0.5f * (gq[i] - gq[o]);
+    const float A = omp * (gp - sym);
+    const float B = omm * (gm - asym);
+    const float F = (float)guo_term<Q>(a, i);   // F_opp = -F
+    const float vi = gq[i] - A - B + F;
+    const float vo = gq[o] - A + B - F;
+    out[i] = solid ? gq[o] : vi;
+    out[o] = solid ? gq[i] : vo;
+  }
+}
+
+}  // namespace prt

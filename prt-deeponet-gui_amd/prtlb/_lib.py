@@ -42,6 +42,23 @@ class AdrParams(ctypes.Structure):
     ]
 
 
+class LbmSegment(ctypes.Structure):
+    """Mirror of prt_lbm_segment."""
+    _fields_ = [("solid", ctypes.c_void_p), ("f_a", ctypes.c_void_p), ("f_b", ctypes.c_void_p),
+                ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("phase", ctypes.c_int32), ("omega_p", ctypes.c_double),
+                ("omega_m", ctypes.c_double), ("force", ctypes.c_double),
+                ("fp16_scale", ctypes.c_double)]
+
+
+class AdrSegment(ctypes.Structure):
+    """Mirror of prt_adr_segment."""
+    _fields_ = [("params", ctypes.POINTER(AdrParams)), ("nx", ctypes.c_int32),
+                ("ny", ctypes.c_int32), ("nz", ctypes.c_int32), ("phase", ctypes.c_int32),
+                ("flags", ctypes.c_void_p), ("u", ctypes.c_void_p), ("c_a", ctypes.c_void_p),
+                ("c_b", ctypes.c_void_p)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _I64 = ctypes.c_int64
@@ -56,11 +73,19 @@ _SIGS = {
     "prt_lbm_partials_count": (_I64, [_I, _I, _I, _I]),
     "prt_lbm_run": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _I, _D, _D, _D, _D, _I, _P, _P]),
     "prt_sum_f64": (_I, [_P, _I64, _P, _P]),
-    "prt_lbm_velocity": (_I, [_I, _I, _I, _I, _I, _P, _P, _D, _D, _P, _P, _P, _P]),
+    "prt_lbm_velocity": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _I, _D, _D, _P, _P, _P, _P]),
     "prt_plane_flux": (_I, [_P, _I, _I, _I, _P, _P]),
     "prt_adr_step": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _P]),
     "prt_adr_step_f64": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _P]),
     "prt_adr_run": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _I, _P]),
+    "prt_lbm_batch_desc_size": (_I64, []),
+    "prt_adr_batch_desc_size": (_I64, []),
+    "prt_lbm_batch_plan": (_I, [_I, _P, _I, _P, _P, _I64, ctypes.POINTER(_I64)]),
+    "prt_adr_batch_plan": (_I, [_P, _I, _P, _P, _I64, ctypes.POINTER(_I64),
+                                ctypes.POINTER(ctypes.c_int)]),
+    "prt_lbm_batch_step": (_I, [_I, _I, _P, _P, _I64, _I, _I, _P, _P]),
+    "prt_adr_batch_step": (_I, [_P, _P, _I64, _I, _I, _P, _P]),
+    "prt_seg_sums": (_I, [_P, _P, _I, _I, _P, _P, _P]),
     "prt_pairwise_plan": (_I, [_I64, _P, _I64, ctypes.POINTER(_I64),
                                ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "prt_pairwise_sum_f32": (_I, [_P, _I64, _I, _P, _I64, _P, _I64, _I64, _I64, _P, _P, _P]),

@@ -348,6 +348,112 @@ def adr_host_setup(mat, vel, pe, da, n_species=2, chem=None, da_abio=0.0,
     return T
 
 
+class AdrRun:
+    """The reference's time loop around a TransportSolver, cut at events.
+
+    3D/tools/prtlb_3d.py:1460-1595: record at the geometric ladder (ratio
+    1.12), check every max(int(1/dt), 100) steps with the per-chemical
+    geometric-tail stop rule, then pick the n_t snapshots.  The device is
+    advanced by whoever owns it (one solver, or a batch of them); this object
+    only says how far it may go before the next event and what to do there.
+    """
+
+    def __init__(self, T, n_t, steps=None, tol=2e-3, max_steps=100000, ratio=1.12,
+                 on_progress=None):
+        self.T, self.n_t, self.steps, self.tol = T, int(n_t), steps, float(tol)
+        self.max_steps, self.ratio, self.on_progress = int(max_steps), ratio, on_progress
+        self.limit = int(steps) if steps is not None else self.max_steps
+        self.keep_only = None
+        if steps is not None:
+            lad = ladder(self.limit, ratio)
+            self.keep_only = {lad[i] for i in pick_snapshots(lad, max(self.limit, 1), self.n_t)}
+        self.keep_steps, self.keep = [], []
+        self.check = max(int(1.0 / float(T.dt)), 100)
+        ns = T.ns
+        self.prev = np.zeros(ns)
+        self.scale = np.full(ns, 1e-6)
+        self.prev_change = np.zeros(ns)
+        self.n = 0
+        self.nxt = 1
+        self.finished = self.limit <= 0      # the reference's loop never runs
+        self._record(0)
+
+    def _record(self, s):
+        self.keep_steps.append(s)
+        self.keep.append(self.T.snapshot() if (self.keep_only is None or s in self.keep_only)
+                         else None)
+
+    def next_target(self):
+        """The step the device may run to before this run needs attention."""
+        nxt_check = (self.n // self.check + 1) * self.check
+        return min(self.limit, self.nxt, nxt_check)
+
+    def arrive(self, n):
+        """The device field is now at step n (== next_target()); handle the
+        record / check there.  Sets self.finished when the run is over."""
+        self.n = n
+        if n >= self.nxt:
+            self._record(n)
+            self.nxt = max(n + 1, int(np.ceil(n * self.ratio)))
+        if n % self.check == 0:
+            cur = self.T.pore_means()
+            self.scale = np.maximum(self.scale, np.abs(cur))
+            change = float((np.abs(cur - self.prev) / self.scale).max())
+            step_ch = np.abs(cur - self.prev) / self.scale
+            with np.errstate(divide="ignore", invalid="ignore"):
+                r = np.where(self.prev_change > 0, step_ch / self.prev_change, 0.0)
+            rest = np.where(r < 1.0, step_ch * r / np.maximum(1.0 - r, 1e-12), np.inf)
+            rest = float(np.nanmax(np.where(np.isfinite(rest), rest, np.inf)))
+            if self.on_progress is not None:
+                self.on_progress(n, self.limit, float(cur[0]),
+                                 rest if np.isfinite(rest) else change)
+            if self.steps is None and n >= 10 * self.check and rest <= self.tol:
+                self.finished = True
+                return
+            self.prev, self.prev_change = cur, step_ch
+        if n >= self.limit:
+            self.finished = True
+
+    def result(self, info=None, da=0.0, da_abio=0.0, vel=None):
+        """(out, t_norm) exactly as the reference assembles them, plus info."""
+        T, n, ns, n_t = self.T, self.n, self.T.ns, self.n_t
+        hit_cap = self.steps is None and n >= self.max_steps
+        if self.keep_steps[-1] != n:
+            self._record(n)
+        total = int(max(n, 1))
+        picked = pick_snapshots(self.keep_steps, total, n_t)
+        repeated = n_t - len(set(picked))
+        out = np.zeros((n_t, ns) + T.shape, T.R)
+        for j, i in enumerate(picked):
+            snap = self.keep[i]
+            if snap is None:
+                raise PrtError("internal: snapshot %d was not kept" % self.keep_steps[i])
+            out[j] = snap.reshape((ns,) + T.shape).cpu().numpy()
+        ks = np.asarray(self.keep_steps)
+        t_norm = (ks[picked].astype(np.float32) / total).astype(np.float32)
+        if info is not None:
+            nd = T.nd
+            info["converged"] = not (hit_cap or self.steps is not None)
+            info["steps"] = total
+            info["dt"] = float(T.dt)
+            info["ladder"] = len(ks)
+            info["chemistry"] = T.ch.source
+            info["da_bio"] = float(da)
+            info["da_abio"] = float(da_abio)
+            info["dt_limited_by"] = ("advection" if T.umax > 2.0 * nd * T.dmax
+                                     else "diffusion")
+            info["biofilm_voxels"] = int(T.biofilm.sum())
+            info["grid_peclet"] = float(T.pe_grid)
+            info["l_ref_voxels"] = float(T.l_ref)
+            info["surface_voxels"] = float(T.surface_voxels)
+            info["repeated_snapshots"] = int(max(repeated, 0))
+            oc = T.outlet_concentration(vel)
+            info["outlet_concentration"] = oc["mean"]
+            info["outlet_flux_concentration"] = oc.get("flux")
+        self.keep = []
+        return out, t_norm
+
+
 def solve_adr(mat, vel, pe, da, n_t, n_species=2, steps=None, tol=2e-3,
               max_steps=100000, info=None, ratio=1.12, chem=None, da_abio=0.0,
               on_progress=None, *, precision="fp32", sorption=None, device=None,
@@ -362,81 +468,13 @@ def solve_adr(mat, vel, pe, da, n_t, n_species=2, steps=None, tol=2e-3,
     per chemical) and outlet_flux_concentration."""
     T = TransportSolver(mat, vel, pe, da, n_species, chem, da_abio,
                         precision=precision, sorption=sorption, device=device)
-    ns = T.ns
-    limit = int(steps) if steps is not None else max_steps
-    keep_only = None
-    if steps is not None:
-        lad = ladder(limit, ratio)
-        keep_only = {lad[i] for i in pick_snapshots(lad, max(limit, 1), n_t)}
-    keep_steps, keep = [], []
-
-    def record(s):
-        keep_steps.append(s)
-        keep.append(T.snapshot() if (keep_only is None or s in keep_only) else None)
-
-    check = max(int(1.0 / float(T.dt)), 100)
-    prev = np.zeros(ns)
-    scale = np.full(ns, 1e-6)
-    prev_change = np.zeros(ns)
-    n, hit_cap = 0, False
-    record(0)
-    nxt = 1
-    while n < limit:
-        nxt_check = (n // check + 1) * check
-        target = min(limit, nxt, nxt_check)
-        T.step_n(target - n)
-        n = target
-        if n >= nxt:
-            record(n)
-            nxt = max(n + 1, int(np.ceil(n * ratio)))
-        if n % check == 0:
-            cur = T.pore_means()
-            scale = np.maximum(scale, np.abs(cur))
-            change = float((np.abs(cur - prev) / scale).max())
-            step_ch = np.abs(cur - prev) / scale
-            with np.errstate(divide="ignore", invalid="ignore"):
-                r = np.where(prev_change > 0, step_ch / prev_change, 0.0)
-            rest = np.where(r < 1.0, step_ch * r / np.maximum(1.0 - r, 1e-12), np.inf)
-            rest = float(np.nanmax(np.where(np.isfinite(rest), rest, np.inf)))
-            if on_progress is not None:
-                on_progress(n, limit, float(cur[0]), rest if np.isfinite(rest) else change)
-            if steps is None and n >= 10 * check and rest <= tol:
-                break
-            prev, prev_change = cur, step_ch
-    if steps is None and n >= max_steps:
-        hit_cap = True
-    if keep_steps[-1] != n:
-        record(n)
-    total = int(max(n, 1))
-
-    picked = pick_snapshots(keep_steps, total, n_t)
-    repeated = n_t - len(set(picked))
-    out = np.zeros((n_t, ns) + T.shape, T.R)
-    for j, i in enumerate(picked):
-        snap = keep[i]
-        if snap is None:
-            raise PrtError("internal: snapshot %d was not kept" % keep_steps[i])
-        out[j] = snap.reshape((ns,) + T.shape).cpu().numpy()
-    ks = np.asarray(keep_steps)
-    t_norm = (ks[picked].astype(np.float32) / total).astype(np.float32)
-    if info is not None:
-        nd = T.nd
-        info["converged"] = not (hit_cap or steps is not None)
-        info["steps"] = total
-        info["dt"] = float(T.dt)
-        info["ladder"] = len(ks)
-        info["chemistry"] = T.ch.source
-        info["da_bio"] = float(da)
-        info["da_abio"] = float(da_abio)
-        info["dt_limited_by"] = ("advection" if T.umax > 2.0 * nd * T.dmax else "diffusion")
-        info["biofilm_voxels"] = int(T.biofilm.sum())
-        info["grid_peclet"] = float(T.pe_grid)
-        info["l_ref_voxels"] = float(T.l_ref)
-        info["surface_voxels"] = float(T.surface_voxels)
-        info["repeated_snapshots"] = int(max(repeated, 0))
-        oc = T.outlet_concentration(vel)
-        info["outlet_concentration"] = oc["mean"]
-        info["outlet_flux_concentration"] = oc.get("flux")
+    run = AdrRun(T, n_t, steps=steps, tol=tol, max_steps=max_steps, ratio=ratio,
+                 on_progress=on_progress)
+    while not run.finished:
+        target = run.next_target()
+        T.step_n(target - run.n)
+        run.arrive(target)
+    out, t_norm = run.result(info, da, da_abio, vel)
     if return_state:
         return out, t_norm, T
     return out, t_norm

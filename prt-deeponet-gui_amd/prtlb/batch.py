@@ -1,0 +1,299 @@
+"""Many geometries per launch: batched Stokes and transport.
+
+LbmBatch and AdrBatch hold a changing set of members (one geometry each)
+and advance all of them with one kernel launch per iteration
+(prt_lbm_batch_step / prt_adr_batch_step).  Members join and leave between
+launches, so a caller can keep the device full while solves of very
+different sizes start and finish -- continuous batching.  A member's result
+is bit-identical to the same solve alone: the per-node arithmetic is the
+single-geometry kernels' own, and every per-member reduction runs over that
+member's nodes only, in a fixed order.
+
+The reference solves one geometry per process (3D/tools/build_dataset_3d.py
+:685-811 loops rocks and parameter sets); this module is the GPU-native
+replacement for that loop's inner solves.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import (PRT_LBM_SKIP_SOLID, PRT_STORE, AdrSegment, LbmSegment,
+                   PrtError, call, lib)
+from .adr import AdrRun, TransportSolver
+from .device import device as _device
+from .device import ptr, stream_ptr
+from .geometry import PORE
+from .stokes import DTYPES, FlowConvergence, checked_flow, trt_minus
+
+_STORE_BYTES = {"fp64": 8, "fp32": 4, "fp16": 2}
+
+
+def lbm_footprint(shape, storage):
+    """Device bytes one Stokes member holds (two population sets + mask)."""
+    n = int(np.prod(shape))
+    q = 9 if len(shape) == 2 else 19
+    return n * (2 * q * _STORE_BYTES[storage] + 1)
+
+
+def adr_footprint(shape, ns, nd):
+    """Device bytes one transport member holds (two fields, u, flags) plus
+    a generous allowance for its snapshots (kept on the device)."""
+    n = int(np.prod(shape))
+    return n * (2 * ns * 4 + nd * 4 + 1) + 12 * n * ns * 4
+
+
+class _Seg:
+    pass
+
+
+def _width(ns):
+    return 1 if ns <= 1 else (2 if ns == 2 else (4 if ns <= 4 else 8))
+
+
+class LbmBatch:
+    """Stokes members on one lattice (q = 9 or 19), advanced together.
+
+    Every member checks FlowConvergence on the same launches (every `every`
+    iterations of its own), because members join only at those boundaries.
+    """
+
+    def __init__(self, q, storage="fp32", skip_solid=True, every=50, device=None,
+                 fp16_scale=1024.0):
+        if q not in (9, 19):
+            raise ValueError("q must be 9 or 19")
+        self.q, self.storage, self.st = q, storage, PRT_STORE[storage]
+        self.flags = PRT_LBM_SKIP_SOLID if skip_solid else 0
+        self.every = int(every)
+        self.dev = _device(device)
+        self.fp16_scale = float(fp16_scale)
+        self.members = {}
+        self.parity = 0
+        self._dirty = True
+        self.n_blocks = 0
+
+    def __len__(self):
+        return len(self.members)
+
+    def bytes(self):
+        return sum(m.bytes for m in self.members.values())
+
+    def add(self, key, g, force=2e-6, tau_lb=0.8, magic=3.0 / 16.0, tol=1e-6, nit=30000):
+        """Admit one geometry (its rest state); it steps from the next launch."""
+        if key in self.members:
+            raise KeyError("member %r already present" % (key,))
+        g = np.asarray(g)
+        nd = 2 if self.q == 9 else 3
+        if g.ndim != nd:
+            raise ValueError("a q=%d batch takes %dD grids" % (self.q, nd))
+        if nit % self.every:
+            raise ValueError("nit must be a multiple of %d in a batch" % self.every)
+        m = _Seg()
+        m.key, m.shape, m.n = key, g.shape, g.size
+        m.dims = (g.shape[0], g.shape[1], 1 if nd == 2 else g.shape[2])
+        m.solid = torch.from_numpy(np.ascontiguousarray(g != PORE).astype(np.uint8).reshape(-1)).to(self.dev)
+        m.n_fluid = int(g.size - int((g != PORE).sum()))
+        m.force, m.tau = float(force), float(tau_lb)
+        m.om_p = 1.0 / m.tau
+        m.om_m = trt_minus(m.om_p, magic)
+        m.f = [torch.empty((self.q, m.n), dtype=DTYPES[self.storage], device=self.dev)
+               for _ in range(2)]
+        s = stream_ptr(self.dev)
+        for buf in m.f:
+            call("prt_lbm_fill_rest", self.q, self.st, m.n, ptr(buf), s)
+        m.phase = self.parity & 1            # newest = f[0] at the current parity
+        m.iters = 0
+        m.nit = int(nit)
+        m.conv = FlowConvergence(tol=tol, every=self.every, max_it=nit,
+                                 name="%dD flow" % nd)
+        m.bytes = lbm_footprint(g.shape, self.storage)
+        self.members[key] = m
+        self._dirty = True
+        return m
+
+    def remove(self, key):
+        self.members.pop(key)
+        self._dirty = True
+
+    def newest(self, m):
+        return m.f[0] if ((self.parity ^ m.phase) & 1) == 0 else m.f[1]
+
+    def _plan(self):
+        mem = list(self.members.values())
+        segs = (LbmSegment * len(mem))()
+        for i, m in enumerate(mem):
+            segs[i] = LbmSegment(ptr(m.solid), ptr(m.f[0]), ptr(m.f[1]), *m.dims, m.phase,
+                                 m.om_p, m.om_m, m.force, self.fp16_scale)
+        dsz = lib().prt_lbm_batch_desc_size()
+        desc = np.zeros(dsz * len(mem), np.uint8)
+        nb = ctypes.c_int64()
+        call("prt_lbm_batch_plan", self.q, segs, len(mem), desc.ctypes.data, None, 0,
+             ctypes.byref(nb))
+        blk = np.zeros(nb.value, np.int32)
+        call("prt_lbm_batch_plan", self.q, segs, len(mem), desc.ctypes.data, blk.ctypes.data,
+             nb.value, ctypes.byref(nb))
+        self.order = mem
+        self.desc = torch.from_numpy(desc).to(self.dev)
+        self.blk_seg = torch.from_numpy(blk).to(self.dev)
+        self.n_blocks = nb.value
+        blk0 = np.zeros(len(mem), np.int32)
+        nblk = np.zeros(len(mem), np.int32)
+        for i in range(1, len(mem)):
+            blk0[i] = blk0[i - 1] + (mem[i - 1].n + 255) // 256
+        for i, m in enumerate(mem):
+            nblk[i] = (m.n + 255) // 256
+        self.blk0 = torch.from_numpy(blk0).to(self.dev)
+        self.nblk = torch.from_numpy(nblk).to(self.dev)
+        self.partials = torch.zeros(self.n_blocks, dtype=torch.float64, device=self.dev)
+        self.sums = torch.zeros(len(mem), dtype=torch.float64, device=self.dev)
+        self._dirty = False
+
+    def step(self, steps, reduce_last=False):
+        """Advance every member `steps` iterations (one launch each)."""
+        if not self.members or steps <= 0:
+            return
+        if self._dirty:
+            self._plan()
+        s = stream_ptr(self.dev)
+        for k in range(steps):
+            red = reduce_last and k == steps - 1
+            call("prt_lbm_batch_step", self.q, self.st, ptr(self.desc), ptr(self.blk_seg),
+                 self.n_blocks, self.parity & 1, self.flags,
+                 ptr(self.partials) if red else None, s)
+            self.parity += 1
+        for m in self.members.values():
+            m.iters += steps
+
+    def tick(self):
+        """`every` iterations, then the convergence check of every member.
+        Returns the members that finished (converged or hit their cap)."""
+        self.step(self.every, reduce_last=True)
+        call("prt_seg_sums", ptr(self.blk0), ptr(self.nblk), len(self.order), 1,
+             ptr(self.partials), ptr(self.sums), stream_ptr(self.dev))
+        sums = self.sums.cpu().numpy()
+        done = []
+        for i, m in enumerate(self.order):
+            um = sums[i] / m.n_fluid if m.n_fluid else 0.0
+            if m.conv.done(m.iters - 1, um) or m.iters >= m.nit:
+                done.append(m)
+        return done
+
+    def finish(self, m, info=None):
+        """Velocity of a finished member (float32 (3, *grid) device tensor,
+        zero in solid) and its info; removes it from the batch."""
+        m.conv.report(info)
+        v32 = torch.empty((3, m.n), dtype=torch.float32, device=self.dev)
+        part = torch.zeros(lib().prt_lbm_partials_count(self.q, *m.dims), dtype=torch.float64,
+                           device=self.dev)
+        out = torch.zeros(1, dtype=torch.float64, device=self.dev)
+        s = stream_ptr(self.dev)
+        newest = self.newest(m)
+        other = m.f[1] if newest is m.f[0] else m.f[0]
+        call("prt_lbm_velocity", self.q, self.st, *m.dims, ptr(m.solid), ptr(newest), ptr(other),
+             self.flags, m.force, self.fp16_scale, None, ptr(v32), ptr(part), s)
+        call("prt_sum_f64", ptr(part), part.numel(), ptr(out), s)
+        sux = float(out.item())
+        v32 = v32.reshape((3,) + m.shape)
+        checked_flow(v32, m.solid.reshape(m.shape).bool(), m.tau, m.force)
+        if info is not None:
+            nu = (m.tau - 0.5) / 3.0
+            info["permeability"] = nu * (sux / m.n) / m.force
+            info["porosity"] = m.n_fluid / m.n
+            info["storage"] = self.storage
+        self.remove(m.key)
+        return v32
+
+
+class AdrBatch:
+    """Float32 transport members advanced together, each with its own event
+    schedule (snapshot ladder, convergence checks) from AdrRun."""
+
+    def __init__(self, device=None):
+        self.dev = _device(device)
+        self.members = {}
+        self.parity = 0
+        self._dirty = True
+
+    def __len__(self):
+        return len(self.members)
+
+    def bytes(self):
+        return sum(m.bytes for m in self.members.values())
+
+    def add(self, key, mat, vel, pe, da, n_t, n_species=2, steps=None, tol=2e-3,
+            max_steps=100000, ratio=1.12, chem=None, da_abio=0.0, sorption=None):
+        if key in self.members:
+            raise KeyError("member %r already present" % (key,))
+        T = TransportSolver(mat, vel, pe, da, n_species, chem, da_abio, precision="fp32",
+                            sorption=sorption, device=self.dev)
+        m = _Seg()
+        m.key, m.T = key, T
+        m.phase = self.parity & 1
+        m.run = AdrRun(T, n_t, steps=steps, tol=tol, max_steps=max_steps, ratio=ratio)
+        m.args = (da, da_abio, vel)
+        m.bytes = adr_footprint(T.shape, T.ns, T.nd)
+        self.members[key] = m
+        self._dirty = True
+        return m
+
+    def remove(self, key):
+        self.members.pop(key)
+        self._dirty = True
+
+    def _sync_latest(self):
+        for m in self.members.values():
+            m.T.latest = 0 if ((self.parity ^ m.phase) & 1) == 0 else 1
+
+    def _plan(self):
+        mem = list(self.members.values())
+        segs = (AdrSegment * len(mem))()
+        for i, m in enumerate(mem):
+            T = m.T
+            segs[i] = AdrSegment(ctypes.pointer(T.params), *T.dims, m.phase, ptr(T.flags),
+                                 ptr(T.u), ptr(T.c[0]), ptr(T.c[1]))
+        dsz = lib().prt_adr_batch_desc_size()
+        desc = np.zeros(dsz * len(mem), np.uint8)
+        nb = ctypes.c_int64()
+        nsm = ctypes.c_int()
+        call("prt_adr_batch_plan", segs, len(mem), desc.ctypes.data, None, 0, ctypes.byref(nb),
+             ctypes.byref(nsm))
+        blk = np.zeros(nb.value, np.int32)
+        call("prt_adr_batch_plan", segs, len(mem), desc.ctypes.data, blk.ctypes.data, nb.value,
+             ctypes.byref(nb), ctypes.byref(nsm))
+        self.desc = torch.from_numpy(desc).to(self.dev)
+        self.blk_seg = torch.from_numpy(blk).to(self.dev)
+        self.n_blocks = nb.value
+        self.max_species = nsm.value
+        self._dirty = False
+
+    def advance(self):
+        """Run every member to the earliest pending event, handle the events.
+        Returns the members whose runs finished."""
+        if not self.members:
+            return []
+        if self._dirty:
+            self._plan()
+        todo = min(m.run.next_target() - m.run.n for m in self.members.values())
+        s = stream_ptr(self.dev)
+        for _ in range(todo):
+            call("prt_adr_batch_step", ptr(self.desc), ptr(self.blk_seg), self.n_blocks,
+                 self.max_species, self.parity & 1, None, s)
+            self.parity += 1
+        self._sync_latest()
+        done = []
+        for m in list(self.members.values()):
+            n = m.run.n + todo
+            m.T.steps_done = n
+            if n == m.run.next_target():
+                m.run.arrive(n)
+            else:
+                m.run.n = n
+            if m.run.finished:
+                done.append(m)
+        return done
+
+    def finish(self, m, info=None):
+        da, da_abio, vel = m.args
+        out = m.run.result(info, da, da_abio, vel)
+        self.remove(m.key)
+        return out

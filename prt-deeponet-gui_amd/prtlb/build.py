@@ -23,6 +23,7 @@ SOURCES = {
     "prt_lbm.hip": [],
     "prt_adr.hip": ["-ffp-contract=off"],
     "prt_reduce.hip": ["-ffp-contract=off"],
+    "prt_batch.hip": [],
 }
 
 

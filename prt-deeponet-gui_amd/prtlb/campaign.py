@@ -1,0 +1,178 @@
+"""Campaign driver: what build_dataset_2d/3d.py do around the solver.
+
+Per rock: a probe Stokes solve at force 1e-6, the force lowered (never
+raised) so the peak speed sits at Mach 0.02, the real Stokes solve, the
+axial pore mean speed; per (rock, Pe, Da): a transport solve.  Restates the
+caller logic of 3D/tools/build_dataset_3d.py:685-726 and
+3D/tools/build_dataset_2d.py:715-759 (the HDF5 writing is replaced by
+returned arrays; h5py is not part of this build).
+
+Two executions of the same campaign:
+  run_sequential  one solve at a time, stokes_d2q9/3q19 + solve_adr
+  run_batched     every rock's Stokes in one LbmBatch, every transport in
+                  one AdrBatch, members admitted under an HBM budget
+"""
+import time
+
+import numpy as np
+import torch
+
+from . import chemistry as chemmod
+from .adr import solve_adr
+from .batch import AdrBatch, LbmBatch, adr_footprint, lbm_footprint
+from .geometry import PORE, disk_pack, sphere_pack
+from .stokes import stokes_d2q9, stokes_d3q19
+
+MACH_TARGET = 0.02
+PROBE_FORCE = 1e-6
+
+
+def lowered_force(probe_v, seed_force):
+    """The reference's probe rule (3D/tools/build_dataset_3d.py:689-712):
+    lower the seed force so the probe's peak speed would sit at Mach 0.02;
+    never raise it."""
+    peak = float(np.sqrt((np.asarray(probe_v, np.float64) ** 2).sum(0)).max())
+    if peak <= 0:
+        return float(seed_force)
+    want = PROBE_FORCE * (MACH_TARGET / np.sqrt(3.0)) / peak
+    return float(want) if want < seed_force else float(seed_force)
+
+
+def axial_mean(v, g):
+    """Axial pore-mean speed, falling back to the mean speed for a closed
+    or recirculating field (3D/tools/build_dataset_3d.py:720-724)."""
+    v = np.asarray(v)
+    m = np.asarray(g) == PORE
+    sp = np.sqrt((v[:3] ** 2).sum(0))
+    ax = float(v[0][m].mean())
+    spd = float(sp[m].mean())
+    return ax if ax > 0.1 * spd else spd
+
+
+def peclet_from_flow(u_lattice, l_ref_voxels, d_lattice):
+    """Pe = u L / D in lattice units (3D/tools/settings_and_units.py:1349-1383)."""
+    return float(u_lattice) * float(l_ref_voxels) / max(float(d_lattice), 1e-30)
+
+
+def pressure_for_peclet(delta_p, u_lattice, peclet, l_ref_voxels, d_lattice):
+    """The pressure gradient giving a wanted Peclet on this rock: Stokes flow
+    is linear, so seed * wanted / seed Peclet (settings_and_units.py:1396-1413)."""
+    seed = peclet_from_flow(u_lattice, l_ref_voxels, d_lattice)
+    return 0.0 if seed <= 0 else float(delta_p) * float(peclet) / seed
+
+
+def campaign_specs(n_geom=64, edge=(48, 160), radius=(3.0, 8.0), porosity=(0.25, 0.5),
+                   pe=(0.1, 100.0), da=(0.01, 10.0), seed=0, nd=3):
+    """Seeded rocks and parameter sets of the batched-campaign workload:
+    cubes with edges U{lo..hi}, radii U(lo, hi), porosity U(lo, hi), Pe and Da
+    log-uniform."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(n_geom):
+        e = int(rng.integers(edge[0], edge[1] + 1))
+        specs.append(dict(
+            key=i, shape=(e,) * nd, radius=tuple(radius),
+            porosity=float(rng.uniform(*porosity)), geom_seed=int(rng.integers(2 ** 31)),
+            pe=float(np.exp(rng.uniform(np.log(pe[0]), np.log(pe[1])))),
+            da=float(np.exp(rng.uniform(np.log(da[0]), np.log(da[1]))))))
+    return specs
+
+
+def make_geometry(spec):
+    if len(spec["shape"]) == 2:
+        return disk_pack(spec["shape"], spec["radius"], spec["porosity"], seed=spec["geom_seed"])
+    return sphere_pack(spec["shape"], spec["radius"], spec["porosity"], seed=spec["geom_seed"])
+
+
+def _adr_kw(spec, kinetics):
+    if kinetics == "sorption_irreversible":
+        return dict(chem=chemmod.sorption_irreversible(True), n_species=2, da=0.0,
+                    da_abio=spec["da"])
+    if kinetics == "sorption_reversible":
+        return dict(chem=chemmod.sorption_reversible(True), n_species=2, da=0.0,
+                    sorption=dict(da_sorb=spec["da"], da_desorb=0.5 * spec["da"]))
+    if kinetics == "monod":
+        return dict(chem=chemmod.monod(0.5), n_species=1, da=spec["da"])
+    return dict(chem=chemmod.first_order(True), n_species=1, da=0.0, da_abio=spec["da"])
+
+
+def run_sequential(specs, geoms, storage="fp64", tau=0.8, tol=1e-6, nit=30000,
+                   kinetics="sorption_irreversible", n_t=5, adr_steps=None, max_steps=100000):
+    """One solve after another (the reference's own schedule, on the GPU)."""
+    res = {}
+    t0 = time.perf_counter()
+    for sp in specs:
+        g = geoms[sp["key"]]
+        fn = stokes_d2q9 if g.ndim == 2 else stokes_d3q19
+        probe = fn(g, nit=nit, force=PROBE_FORCE, tau_lb=tau, tol=tol, storage=storage)
+        force = lowered_force(probe, 2e-6)
+        info = {}
+        v = fn(g, nit=nit, force=force, tau_lb=tau, tol=tol, info=info, storage=storage)
+        ainfo = {}
+        kw = _adr_kw(sp, kinetics)
+        out, tn = solve_adr(g, v, sp["pe"], kw.pop("da"), n_t, steps=adr_steps,
+                            max_steps=max_steps, info=ainfo, **kw)
+        res[sp["key"]] = dict(permeability=info["permeability"], flow_iters=info["flow_iters"],
+                              force=force, adr_steps=ainfo["steps"],
+                              outlet=np.asarray(ainfo["outlet_concentration"]).tolist(),
+                              out_last=out[-1])
+    torch.cuda.synchronize()
+    return res, time.perf_counter() - t0
+
+
+def run_batched(specs, geoms, storage="fp32", tau=0.8, tol=1e-6, nit=30000,
+                kinetics="sorption_irreversible", n_t=5, adr_steps=None, max_steps=100000,
+                hbm_budget=200e9):
+    """Every Stokes solve in one LbmBatch per lattice, every transport solve
+    in one AdrBatch; members admitted while they fit the HBM budget."""
+    res = {sp["key"]: {} for sp in specs}
+    t0 = time.perf_counter()
+    nd = len(specs[0]["shape"])
+    q = 9 if nd == 2 else 19
+    # ---- Stokes: probe stage then real stage, one batch
+    lb = LbmBatch(q, storage=storage)
+    pending = [(sp["key"], "probe") for sp in specs]
+    velocities = {}
+    while pending or len(lb):
+        while pending:
+            key, stage = pending[0]
+            g = geoms[key]
+            if len(lb) and lb.bytes() + lbm_footprint(g.shape, storage) > hbm_budget:
+                break
+            pending.pop(0)
+            force = PROBE_FORCE if stage == "probe" else res[key]["force"]
+            m = lb.add((key, stage), g, force=force, tau_lb=tau, tol=tol, nit=nit)
+            m.stage = stage
+        for m in lb.tick():
+            key, stage = m.key
+            info = {}
+            v = lb.finish(m, info).cpu().numpy()
+            if stage == "probe":
+                res[key]["force"] = lowered_force(v, 2e-6)
+                pending.append((key, "real"))
+            else:
+                res[key]["permeability"] = info["permeability"]
+                res[key]["flow_iters"] = m.conv.iters
+                velocities[key] = v
+    # ---- transport: one batch
+    ab = AdrBatch()
+    queue = list(specs)
+    while queue or len(ab):
+        while queue:
+            sp = queue[0]
+            g = geoms[sp["key"]]
+            kw = _adr_kw(sp, kinetics)
+            if len(ab) and ab.bytes() + adr_footprint(g.shape, kw["n_species"], g.ndim) > hbm_budget:
+                break
+            queue.pop(0)
+            ab.add(sp["key"], g, velocities[sp["key"]], sp["pe"], kw.pop("da"), n_t,
+                   steps=adr_steps, max_steps=max_steps, **kw)
+        for m in ab.advance():
+            ainfo = {}
+            out, tn = ab.finish(m, ainfo)
+            r = res[m.key]
+            r.update(adr_steps=ainfo["steps"],
+                     outlet=np.asarray(ainfo["outlet_concentration"]).tolist(),
+                     out_last=out[-1])
+    torch.cuda.synchronize()
+    return res, time.perf_counter() - t0

@@ -1,0 +1,159 @@
+"""A solve-request server with continuous batching.
+
+Requests (a pore mask, Pe, Da, kinetics) arrive at any time.  Each one is
+admitted as soon as the HBM budget has room for it and then runs through
+two stages -- Stokes (one member of the LbmBatch of its lattice), then
+transport (one member of the AdrBatch) -- while its populations and fields
+stay on the device.  Every scheduling round advances ALL admitted requests
+with one launch per batch, so requests that arrive while others run are
+folded into the same launches as soon as memory frees up; a finished
+request leaves its batch and frees its memory at once.
+
+Each request's result is bit-identical to the same request served alone
+(tests/test_gpu_serve.py checks it): the batched kernels run the
+single-geometry arithmetic, and every per-request reduction covers that
+request's nodes only.  The reference has no serving layer; its solver is a
+per-process function (3D/tools/prtlb_3d.py:348, :781).
+"""
+import time
+from collections import deque
+
+import numpy as np
+import torch
+
+from . import campaign as C
+from .batch import AdrBatch, LbmBatch, adr_footprint, lbm_footprint
+
+
+class Request:
+    def __init__(self, key, g, pe, da, kinetics="sorption_reversible", arrival=0.0,
+                 force=2e-6, probe=False):
+        self.key, self.g, self.pe, self.da = key, np.asarray(g), float(pe), float(da)
+        self.kinetics, self.arrival, self.force, self.probe = kinetics, float(arrival), float(force), probe
+        self.t_admit = self.t_done = None
+        self.result = None
+
+
+class Server:
+    """Continuous-batching PRT-LB server on one GPU."""
+
+    def __init__(self, hbm_budget=200e9, storage="fp32", tau=0.8, tol=1e-6, nit=30000, n_t=5,
+                 adr_steps=None, max_steps=100000, device=None):
+        self.budget = float(hbm_budget)
+        self.storage, self.tau, self.tol, self.nit = storage, tau, tol, nit
+        self.n_t, self.adr_steps, self.max_steps = n_t, adr_steps, max_steps
+        self.lb = {9: LbmBatch(9, storage, device=device), 19: LbmBatch(19, storage, device=device)}
+        self.ab = AdrBatch(device=device)
+        self.waiting = deque()
+        self.done = []
+        self.reserved = {}
+
+    def submit(self, req):
+        self.waiting.append(req)
+
+    def _footprint(self, req):
+        g = req.g
+        kw = C._adr_kw(dict(da=req.da), req.kinetics)
+        return max(lbm_footprint(g.shape, self.storage), adr_footprint(g.shape, kw["n_species"], g.ndim))
+
+    def in_use(self):
+        return sum(self.reserved.values())
+
+    def _admit(self, now):
+        while self.waiting and self.waiting[0].arrival <= now:
+            req = self.waiting[0]
+            need = self._footprint(req)
+            if self.reserved and self.in_use() + need > self.budget:
+                break
+            self.waiting.popleft()
+            self.reserved[req.key] = need
+            req.t_admit = now
+            q = 9 if req.g.ndim == 2 else 19
+            stage = "probe" if req.probe else "real"
+            m = self.lb[q].add((req.key, stage), req.g, force=C.PROBE_FORCE if req.probe else req.force,
+                               tau_lb=self.tau, tol=self.tol, nit=self.nit)
+            m.req = req
+
+    def _round(self, now):
+        self._admit(now)
+        for q, lb in self.lb.items():
+            if not len(lb):
+                continue
+            for m in lb.tick():
+                req = m.req
+                key, stage = m.key
+                info = {}
+                v = lb.finish(m, info).cpu().numpy()
+                if stage == "probe":
+                    req.force = C.lowered_force(v, req.force)
+                    m2 = lb.add((key, "real"), req.g, force=req.force, tau_lb=self.tau,
+                                tol=self.tol, nit=self.nit)
+                    m2.req = req
+                    continue
+                req.permeability = info["permeability"]
+                req.velocity = v
+                kw = C._adr_kw(dict(da=req.da), req.kinetics)
+                am = self.ab.add(key, req.g, v, req.pe, kw.pop("da"), self.n_t,
+                                 steps=self.adr_steps, max_steps=self.max_steps, **kw)
+                am.req = req
+        if len(self.ab):
+            for m in self.ab.advance():
+                req = m.req
+                ainfo = {}
+                out, tn = self.ab.finish(m, ainfo)
+                req.result = dict(out=out, t_norm=tn, info=ainfo, permeability=req.permeability,
+                                  velocity=req.velocity)
+                req.t_done = time.perf_counter()
+                self.reserved.pop(req.key, None)
+                self.done.append(req)
+
+    def idle(self):
+        return not (self.waiting or any(len(b) for b in self.lb.values()) or len(self.ab))
+
+    def run(self, requests, realtime=True):
+        """Serve `requests` (arrival times in seconds from now when realtime,
+        else all at once) until every one is done.  Returns them, with
+        latency = done - arrival."""
+        t0 = time.perf_counter()
+        for r in sorted(requests, key=lambda r: r.arrival):
+            self.submit(r)
+        while not self.idle():
+            now = (time.perf_counter() - t0) if realtime else float("inf")
+            busy = any(len(b) for b in self.lb.values()) or len(self.ab)
+            if not busy and self.waiting and realtime and self.waiting[0].arrival > now:
+                time.sleep(min(0.01, self.waiting[0].arrival - now))
+                continue
+            self._round(now)
+        for r in requests:
+            r.latency = (r.t_done - t0) - (r.arrival if realtime else 0.0)
+        return requests
+
+
+def poisson_requests(n, rate=4.0, frac_2d=0.3, seed=0, size2d=(256, 1024), size3d=(64, 256),
+                     pe=(0.1, 100.0), da=(0.01, 10.0), kinetics="sorption_reversible"):
+    """The serving workload: Poisson arrivals, 30% D2Q9 disk packs and 70%
+    D3Q19 sphere packs of random size, Pe and Da log-uniform."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for i in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        two_d = rng.random() < frac_2d
+        if two_d:
+            e = int(rng.integers(size2d[0], size2d[1] + 1))
+            spec = dict(shape=(e, e), radius=6.0, porosity=float(rng.uniform(0.5, 0.7)))
+        else:
+            e = int(rng.integers(size3d[0], size3d[1] + 1))
+            spec = dict(shape=(e, e, e), radius=(3.0, 8.0), porosity=float(rng.uniform(0.25, 0.5)))
+        spec["geom_seed"] = int(rng.integers(2 ** 31))
+        g = C.make_geometry(spec)
+        p = float(np.exp(rng.uniform(np.log(pe[0]), np.log(pe[1]))))
+        d = float(np.exp(rng.uniform(np.log(da[0]), np.log(da[1]))))
+        reqs.append(Request(i, g, p, d, kinetics=kinetics, arrival=t))
+    return reqs
+
+
+def latency_summary(reqs, wall):
+    lat = np.array([r.latency for r in reqs])
+    return dict(n=len(reqs), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                mean_s=float(lat.mean()), solves_per_hour=len(reqs) / wall * 3600.0, wall_s=wall)

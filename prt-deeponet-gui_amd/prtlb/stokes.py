@@ -178,8 +178,8 @@ class StokesSolver:
         v32 = torch.empty((3, self.n), dtype=torch.float32, device=self.dev)
         v64 = torch.empty((3, self.n), dtype=torch.float64, device=self.dev) if f64 else None
         call("prt_lbm_velocity", self.q, self.st, *self.dims, ptr(self.solid),
-             ptr(self.f[self.latest]), self.force, self.fp16_scale, ptr(v64), ptr(v32),
-             ptr(self.partials), self._stream())
+             ptr(self.f[self.latest]), ptr(self.f[1 - self.latest]), self.flags, self.force,
+             self.fp16_scale, ptr(v64), ptr(v32), ptr(self.partials), self._stream())
         call("prt_sum_f64", ptr(self.partials), self.partials.numel(), ptr(self.scalar),
              self._stream())
         sux = float(self.scalar[0].item())

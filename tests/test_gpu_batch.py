@@ -1,0 +1,102 @@
+"""Batched solves: a member of a batch gives the same bits as the same solve
+alone, and the continuous-batching server returns, for every request, what
+that request gives when served alone."""
+import numpy as np
+import pytest
+
+import prtlb
+from prtlb import campaign as C
+from prtlb.batch import AdrBatch, LbmBatch
+from prtlb.serve import Request, Server
+
+pytestmark = pytest.mark.gpu
+
+
+def test_lbm_batch_equals_single():
+    geoms = [prtlb.sphere_pack((20, 16, 12), (2.0, 3.5), 0.55, seed=s) for s in range(3)]
+    geoms.append(prtlb.sphere_pack((33, 17, 9), (2.0, 3.0), 0.6, seed=9))
+    lb = LbmBatch(19, storage="fp32")
+    for i, g in enumerate(geoms):
+        lb.add(i, g, force=1e-5 * (i + 1), nit=300, tol=0.0)
+    for _ in range(6):
+        lb.tick()
+    for i, g in enumerate(geoms):
+        m = lb.members[i]
+        v_b = lb.finish(m).cpu().numpy()
+        v_s = prtlb.stokes_d3q19(g, nit=300, force=1e-5 * (i + 1), tol=0.0, storage="fp32",
+                                 skip_solid=True)
+        assert np.array_equal(v_b, v_s), i
+
+
+def test_lbm_batch_join_late_and_converge():
+    g1 = prtlb.disk_pack((40, 30), 3.0, 0.6, seed=1)
+    g2 = prtlb.disk_pack((64, 20), 3.0, 0.6, seed=2)
+    lb = LbmBatch(9, storage="fp64", skip_solid=False)
+    lb.add("a", g1, force=2e-6, tol=1e-6, nit=20000)
+    for _ in range(3):
+        lb.tick()
+    lb.add("b", g2, force=2e-6, tol=1e-6, nit=20000)      # joins 150 iterations later
+    got = {}
+    while len(lb):
+        for m in lb.tick():
+            info = {}
+            got[m.key] = (lb.finish(m, info).cpu().numpy(), info)
+    for key, g in (("a", g1), ("b", g2)):
+        info = {}
+        v = prtlb.stokes_d2q9(g, nit=20000, force=2e-6, tol=1e-6, info=info)
+        assert np.array_equal(got[key][0], v)
+        assert got[key][1]["flow_iters"] == info["flow_iters"]
+
+
+def test_adr_batch_equals_single():
+    g1 = prtlb.disk_pack((48, 24), 3.0, 0.65, seed=3)
+    g2 = prtlb.sphere_pack((24, 16, 12), (2.0, 3.0), 0.55, seed=4)
+    v1 = prtlb.stokes_d2q9(g1, nit=2000, force=1e-5, tol=1e-6)
+    v2 = prtlb.stokes_d3q19(g2, nit=1000, force=1e-5, tol=1e-6)
+    ab = AdrBatch()
+    ab.add("a", g1, v1, 20.0, 1.0, 5, n_species=2)                    # auto stop
+    ab.add("b", g2, v2, 5.0, 0.0, 4, n_species=2, steps=700,
+           chem=prtlb.chemistry.sorption_irreversible(), da_abio=2.0)
+    got = {}
+    while len(ab):
+        for m in ab.advance():
+            info = {}
+            got[m.key] = ab.finish(m, info) + (info,)
+    ia, ib = {}, {}
+    oa, ta = prtlb.solve_adr(g1, v1, 20.0, 1.0, 5, n_species=2, info=ia)
+    ob, tb = prtlb.solve_adr(g2, v2, 5.0, 0.0, 4, n_species=2, steps=700, info=ib,
+                             chem=prtlb.chemistry.sorption_irreversible(), da_abio=2.0)
+    assert np.array_equal(got["a"][0], oa) and np.array_equal(got["a"][1], ta)
+    assert got["a"][2]["steps"] == ia["steps"]
+    assert np.array_equal(got["b"][0], ob) and np.array_equal(got["b"][1], tb)
+
+
+def test_server_result_same_alone_and_in_mix():
+    specs = [dict(shape=(96, 96), radius=6.0, porosity=0.6, geom_seed=1),
+             dict(shape=(40, 40, 40), radius=(3.0, 6.0), porosity=0.4, geom_seed=2),
+             dict(shape=(32, 32, 32), radius=(3.0, 6.0), porosity=0.45, geom_seed=3),
+             dict(shape=(128, 128), radius=6.0, porosity=0.6, geom_seed=4)]
+    pes, das = [3.0, 10.0, 0.5, 40.0], [0.1, 1.0, 5.0, 0.3]
+    geoms = [C.make_geometry(s) for s in specs]
+
+    def reqs():
+        return [Request(i, geoms[i], pes[i], das[i], arrival=0.0) for i in range(4)]
+
+    mix = Server(nit=30000, adr_steps=None, max_steps=3000).run(reqs(), realtime=False)
+    for r in mix:
+        alone = Server(nit=30000, adr_steps=None, max_steps=3000).run(
+            [Request(r.key, geoms[r.key], pes[r.key], das[r.key])], realtime=False)[0]
+        assert np.array_equal(r.result["velocity"], alone.result["velocity"])
+        assert np.array_equal(r.result["out"], alone.result["out"])
+        assert r.result["info"]["steps"] == alone.result["info"]["steps"]
+
+
+def test_campaign_batched_matches_sequential_fp32():
+    specs = C.campaign_specs(n_geom=3, edge=(20, 28), seed=5)
+    geoms = {s["key"]: C.make_geometry(s) for s in specs}
+    rb, _ = C.run_batched(specs, geoms, storage="fp32", adr_steps=200)
+    rs, _ = C.run_sequential(specs, geoms, storage="fp32", adr_steps=200)
+    for k in rb:
+        assert rb[k]["flow_iters"] == rs[k]["flow_iters"]
+        assert rb[k]["permeability"] == pytest.approx(rs[k]["permeability"], rel=1e-12)
+        assert np.array_equal(rb[k]["out_last"], rs[k]["out_last"])

@@ -1,0 +1,160 @@
+"""Stokes on the GPU against the reference's golden vectors, the oracle and
+closed forms.  FP64 fields within 1e-12 relative (max |dv| / max |v|)."""
+import contextlib
+import io
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import golden
+from oracle import prtlb_oracle as O
+import prtlb
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def solver(g):
+    return prtlb.stokes_d2q9 if np.asarray(g).ndim == 2 else prtlb.stokes_d3q19
+
+
+@pytest.mark.parametrize("name", ["stokes2d_fixed", "stokes2d_conv",
+                                  "stokes3d_fixed", "stokes3d_conv"])
+def test_golden_fp64(name):
+    d = golden(name)
+    mt = d["meta"]
+    print("case", name, d["g"].shape, flush=True)
+    info = {}
+    v32, v64 = solver(d["g"])(d["g"], nit=mt["nit"], force=mt["force"], tau_lb=mt["tau"],
+                              tol=mt["tol"], info=info, return_f64=True)
+    assert rel(v64, d["v64"]) < 1e-12
+    ulp = np.spacing(np.abs(d["v32"]).astype(np.float32))
+    assert (np.abs(v32.astype(np.float64) - d["v32"]) <= ulp).all()
+    assert info["flow_iters"] == mt["info"]["flow_iters"]
+    assert info["flow_converged"] == mt["info"]["flow_converged"]
+
+
+@pytest.mark.parametrize("shape,seed", [((40, 24), 0), ((20, 16, 12), 1), ((17, 9, 13), 2)])
+def test_vs_oracle_fixed_steps(shape, seed):
+    if len(shape) == 2:
+        g = prtlb.disk_pack(shape, 3.0, 0.6, seed=seed)
+    else:
+        g = prtlb.sphere_pack(shape, (2.0, 3.5), 0.55, seed=seed)
+    print("case", shape, seed, flush=True)
+    nit = 300
+    _, (v64o, _) = O.stokes(g, nit=nit, force=1e-5, tau_lb=0.8, tol=0.0, return_f64=True)
+    v32, v64 = solver(g)(g, nit=nit, force=1e-5, tau_lb=0.8, tol=0.0, return_f64=True)
+    assert rel(v64, v64o) < 1e-12
+    # fluid-only update gives the same field bit for bit
+    _, v64s = solver(g)(g, nit=nit, force=1e-5, tau_lb=0.8, tol=0.0, return_f64=True,
+                        skip_solid=True)
+    assert np.array_equal(v64s, v64)
+
+
+def test_reduced_precision_errors():
+    g = prtlb.disk_pack((64, 48), 4.0, 0.6, seed=5)
+    ref64 = prtlb.stokes_d2q9(g, nit=20000, force=2e-6, tol=1e-10, return_f64=True)[1]
+    errs = {}
+    for st in ("fp32", "fp16"):
+        info = {}
+        v = prtlb.stokes_d2q9(g, nit=20000, force=2e-6, tol=1e-10, info=info, storage=st)
+        errs[st] = rel(v, ref64)
+    print("storage errors vs FP64:", errs, flush=True)
+    assert errs["fp32"] < 1e-4
+    assert errs["fp16"] < 5e-2
+
+
+def _slit2d(H, nx=12):
+    g = np.full((nx, H + 2), prtlb.PORE, np.uint8)
+    g[:, 0] = g[:, -1] = prtlb.SOLID
+    return g
+
+
+def _slit3d(H, nx=8, nz=8):
+    g = np.full((nx, H + 2, nz), prtlb.PORE, np.uint8)
+    g[:, 0, :] = g[:, -1, :] = prtlb.SOLID
+    return g
+
+
+def parabola(H, force, tau):
+    nu = (tau - 0.5) / 3.0
+    y = np.arange(1, H + 1) - 0.5
+    return (force / (2 * nu)) * y * (H - y)
+
+
+def test_poiseuille_2d_3d():
+    v = prtlb.stokes_d2q9(_slit2d(20), nit=40000, force=2e-6, tau_lb=0.8)
+    ux, ex = v[0][6, 1:-1], parabola(20, 2e-6, 0.8)
+    assert abs(ux.max() / ex.max() - 1) < 0.01
+    assert np.abs(ux - ex).max() / ex.max() < 0.015
+    v3 = prtlb.stokes_d3q19(_slit3d(16), nit=40000, force=2e-6, tau_lb=0.8)
+    ux3, ex3 = v3[0][4, 1:-1, 4], parabola(16, 2e-6, 0.8)
+    assert abs(ux3.max() / ex3.max() - 1) < 0.015
+    assert np.abs(ux3 - ex3).max() / ex3.max() < 0.02
+
+
+@pytest.mark.parametrize("tau", [0.55, 0.8, 0.9330127, 1.2, 2.0])
+def test_wall_stays_at_midlink(tau):
+    for H in (4, 8, 20):
+        v = prtlb.stokes_d2q9(_slit2d(H), nit=20000, force=2e-6, tau_lb=tau)
+        ux, ex = v[0][6, 1:-1], parabola(H, 2e-6, tau)
+        assert np.abs(ux - ex).max() / ex.max() < 0.005, (tau, H)
+
+
+def test_d2q9_equals_d3q19_and_slit_permeability():
+    for H in (4, 8):
+        i2, i3 = {}, {}
+        u2 = prtlb.stokes_d2q9(_slit2d(H), nit=20000, force=2e-6, info=i2)[0].max()
+        u3 = prtlb.stokes_d3q19(_slit3d(H), nit=20000, force=2e-6, info=i3)[0].max()
+        assert abs(u2 - u3) / u2 < 1e-3
+        # Darcy permeability of the slit: closed form of the node-sampled parabola
+        k = O.slit_permeability(H)
+        i = {}
+        prtlb.stokes_d2q9(_slit2d(H), nit=200000, force=2e-6, tol=1e-12, info=i)
+        assert abs(i["permeability"] / k - 1) < 1e-6, (i["permeability"], k)
+
+
+def test_mass_conserved_full_size():
+    # Sum of all populations is invariant: collision conserves mass, the Guo
+    # term sums to zero, solid reversal and streaming permute.
+    g = prtlb.sphere_pack((128, 128, 128), (4.0, 8.0), 0.35, seed=0)
+    s = prtlb.StokesSolver(g, force=2e-6)
+    m0 = float(s.populations().sum(dtype=torch.float64))
+    s.run(200)
+    m1 = float(s.populations().sum(dtype=torch.float64))
+    assert abs(m1 - m0) / m0 < 1e-12
+    v32, v64, sux = s.velocity(f64=True)
+    solid = s.solid.reshape(s.shape).bool()
+    assert float(v64[:, solid].abs().max()) == 0.0
+    assert np.isfinite(s.permeability(sux))
+
+
+def test_plane_flux_constant_and_solid_zero():
+    g = np.full((16, 16, 16), prtlb.PORE, np.uint8)
+    g[:, 0, :] = g[:, -1, :] = prtlb.SOLID
+    g[:, :, 0] = g[:, :, -1] = prtlb.SOLID
+    v32, v64 = prtlb.stokes_d3q19(g, nit=400, force=2e-6, return_f64=True, as_tensor=True)
+    flux = prtlb.plane_flux(v64).cpu().numpy()
+    assert (flux.max() - flux.min()) / abs(flux.mean()) < 0.01
+    assert float(v32[:, torch.from_numpy(g != prtlb.PORE).cuda()].abs().max()) == 0.0
+
+
+def test_mach_warning_and_refusals():
+    g = np.full((16, 16, 16), prtlb.PORE, np.uint8)
+    g[:, 0, :] = g[:, -1, :] = prtlb.SOLID
+    buf = io.StringIO()
+    with contextlib.redirect_stdout(buf):
+        prtlb.stokes_d3q19(g, nit=200, force=3e-3, tau_lb=0.8)
+    assert "Mach" in buf.getvalue()
+    with pytest.raises(SystemExit):
+        prtlb.stokes_d2q9(np.zeros((8, 8), np.uint8), nit=200)
+    with pytest.raises(ValueError):
+        prtlb.stokes_d2q9(np.full((8, 8), 2, np.uint8), tau_lb=0.5)
+    with pytest.raises(ValueError):
+        prtlb.stokes_d3q19(np.full((8, 8), 2, np.uint8))
