@@ -18,7 +18,9 @@
  *     layout of the reference's (nx, ny[, nz]) arrays.  A 2D grid is passed
  *     with nz = 1.
  *   - populations are structure-of-arrays: population i of node n lives at
- *     f[i * n_nodes + n].
+ *     f[i * ld + n], ld = prt_lbm_pop_stride(n_nodes, storage) >= n_nodes
+ *     (padded so the Q streams start at staggered HBM offsets); a population
+ *     buffer holds q * ld elements.
  */
 #ifndef PRTLB_H
 #define PRTLB_H
@@ -41,8 +43,12 @@ extern "C" {
 #define PRT_STORE_FP32 1 /* f_i - w_i, float */
 #define PRT_STORE_FP16 2 /* (f_i - w_i) * (s / w_i), IEEE half */
 
-/* prt_lbm_step flags */
+/* prt_lbm_step / prt_lbm_run flags */
 #define PRT_LBM_SKIP_SOLID 1 /* solid nodes are neither read nor written */
+/* launch shape: 2^bz_log2 x 2^by_log2 threads (64..512), npt (1 or 2) nodes
+ * per thread along y; 0 = the library's default shape */
+#define PRT_LBM_TUNE(bz_log2, by_log2, npt) \
+  (((bz_log2) << 8) | ((by_log2) << 12) | (((npt) - 1) << 16))
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *prt_last_error(void);
@@ -54,6 +60,9 @@ const char *prt_version(void);
 size_t prt_adr_params_size(void);
 
 /* ------------------------------------------------------------------ Stokes */
+
+/* Element stride between the population arrays of an n_nodes grid. */
+int64_t prt_lbm_pop_stride(int64_t n_nodes, int storage);
 
 /* Rest state f_i = w_i (zero deviation in the shifted storages) for n_nodes
  * nodes.  Replaces `f = np.ones(...) * w` (3D/tools/prtlb_3d.py:403,
@@ -80,7 +89,7 @@ int prt_lbm_step(int q, int storage, int nx, int ny, int nz,
                  double omega_p, double omega_m, double force,
                  double fp16_scale, int flags, double *partials, void *stream);
 
-/* Number of partial sums prt_lbm_step writes for this grid. */
+/* Length a partials buffer needs for this grid (any launch shape). */
 int64_t prt_lbm_partials_count(int q, int nx, int ny, int nz);
 
 /* `steps` fused iterations alternating f_a -> f_b -> f_a ...; the last one
@@ -204,7 +213,7 @@ int64_t prt_adr_batch_desc_size(void);
 /* Build the descriptors (desc_host: nseg * desc_size bytes, HOST) and the
  * block-to-segment table (blk_seg_host, HOST, NULL = size query into
  * *n_blocks); the caller copies both to the device. */
-int prt_lbm_batch_plan(int q, const prt_lbm_segment *segs, int nseg,
+int prt_lbm_batch_plan(int q, int storage, const prt_lbm_segment *segs, int nseg,
                        void *desc_host, int32_t *blk_seg_host,
                        int64_t blk_capacity, int64_t *n_blocks);
 int prt_adr_batch_plan(const prt_adr_segment *segs, int nseg, void *desc_host,
@@ -230,6 +239,13 @@ int prt_adr_batch_step(const void *desc_dev, const int32_t *blk_seg_dev,
 int prt_seg_sums(const int32_t *blk0_dev, const int32_t *nblk_dev, int nseg,
                  int width, const double *partials, double *out,
                  void *stream);
+
+/* --------------------------------------------------------- measurement */
+
+/* Plain device-to-device copy of `bytes` (multiple of 16, 16-byte aligned
+ * buffers): the STREAM-copy bandwidth ceiling and the known-size kernel for
+ * calibrating HBM traffic counters.  No counterpart in the reference. */
+int prt_stream_copy(const void *src, void *dst, int64_t bytes, void *stream);
 
 /* ------------------------------------------------------ NumPy-order sums */
 

@@ -144,9 +144,9 @@ using namespace prt;
 extern "C" int64_t prt_lbm_batch_desc_size(void) { return (int64_t)sizeof(SegLbm); }
 extern "C" int64_t prt_adr_batch_desc_size(void) { return (int64_t)sizeof(SegAdr); }
 
-extern "C" int prt_lbm_batch_plan(int q, const prt_lbm_segment *segs, int nseg, void *desc_host,
+extern "C" int prt_lbm_batch_plan(int q, int storage, const prt_lbm_segment *segs, int nseg, void *desc_host,
                                   int32_t *blk_seg_host, int64_t blk_capacity, int64_t *n_blocks) {
-  if (!segs || nseg < 1 || !desc_host || !n_blocks || (q != 9 && q != 19)) {
+  if (!segs || nseg < 1 || !desc_host || !n_blocks || (q != 9 && q != 19) || storage < 0 || storage > 2) {
     set_error("prt_lbm_batch_plan: bad arguments");
     return PRT_E_ARG;
   }
@@ -173,7 +173,8 @@ extern "C" int prt_lbm_batch_plan(int q, const prt_lbm_segment *segs, int nseg, 
       a.X = g.nx; a.Y = g.ny; a.Z = g.nz;
     }
     a.N = (long long)g.nx * g.ny * g.nz;
-    if (a.N * q >= (1LL << 31)) {
+    a.ld = pop_stride(a.N, storage == PRT_STORE_FP64 ? 8 : (storage == PRT_STORE_FP32 ? 4 : 2));
+    if (a.ld * q >= (1LL << 31)) {
       set_error("prt_lbm_batch_plan: segment %d too large for the 32-bit index", s);
       return PRT_E_RANGE;
     }

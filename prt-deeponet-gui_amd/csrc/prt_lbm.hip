@@ -20,38 +20,42 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "prt_lbm_core.h"
 
 namespace prt {
 
-template <int Q, int ST, bool SKIP, bool RED>
-__global__ __launch_bounds__(256) void lbm_step_kernel(LbmArgs a) {
+template <int Q, int ST, bool SKIP, bool RED, int NPT>
+__global__ __launch_bounds__(512) void lbm_step_kernel(LbmArgs a) {
   __shared__ double lds[16];
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int y0 = (blockIdx.y * blockDim.y + threadIdx.y) * NPT;
   const int x = blockIdx.z;
-  const bool inside = z < a.Z && y < a.Y;
   double contrib = 0.0;
-  if (inside) {
-    const int n = (x * a.Y + y) * a.Z + z;
-    const bool solid = a.solid[n] != 0;
-    if (!(SKIP && solid)) {
-      const Nbr nb = make_nbr(x, y, z, a.X, a.Y, a.Z);
-      double ux;
-      if constexpr (ST == PRT_STORE_FP64) {
-        double f[Q], fp[Q];
-        gather<Q, ST, SKIP>(a, nb, n, f);
-        collide_ref<Q>(a, solid, f, fp, ux);
-        scatter<Q, ST>(a, n, fp);
-      } else {
-        float f[Q], fp[Q];
-        gather<Q, ST, SKIP>(a, nb, n, f);
-        collide_dev<Q>(a, solid, f, fp, ux);
-        scatter<Q, ST>(a, n, fp);
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const int y = y0 + k;
+    if (z < a.Z && y < a.Y) {
+      const int n = (x * a.Y + y) * a.Z + z;
+      const bool solid = a.solid[n] != 0;
+      if (!(SKIP && solid)) {
+        const Nbr nb = make_nbr(x, y, z, a.X, a.Y, a.Z);
+        double ux;
+        if constexpr (ST == PRT_STORE_FP64) {
+          double f[Q], fp[Q];
+          gather<Q, ST, SKIP>(a, nb, n, f);
+          collide_ref<Q>(a, solid, f, fp, ux);
+          scatter<Q, ST>(a, n, fp);
+        } else {
+          float f[Q], fp[Q];
+          gather<Q, ST, SKIP>(a, nb, n, f);
+          collide_dev<Q>(a, solid, f, fp, ux);
+          scatter<Q, ST>(a, n, fp);
+        }
+        if (RED && !solid) contrib += fabs(ux);
       }
-      if (RED && !solid) contrib = fabs(ux);
     }
   }
   if (RED) {
@@ -130,12 +134,12 @@ __global__ __launch_bounds__(256) void lbm_velocity_kernel(LbmArgs a, double *v6
 }
 
 template <int Q, int ST>
-__global__ void fill_rest_kernel(void *f, long long N) {
+__global__ void fill_rest_kernel(void *f, long long ld) {
   using T = typename Pop<ST>::T;
   T *p = static_cast<T *>(f);
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < N * Q; k += stride) {
-    const int i = (int)(k / N);
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < ld * Q; k += stride) {
+    const int i = (int)(k / ld);
     if constexpr (ST == PRT_STORE_FP64)
       p[k] = weight<Q>(i);
     else
@@ -177,9 +181,10 @@ struct Geom {
   int X, Y, Z;
   long long N;
   dim3 block, grid;
+  int npt;
 };
 
-static int make_geom(int q, int nx, int ny, int nz, Geom &g) {
+static int make_geom(int q, int nx, int ny, int nz, Geom &g, int flags = 0) {
   if (q != 9 && q != 19) {
     set_error("q must be 9 (D2Q9) or 19 (D3Q19), got %d", q);
     return PRT_E_ARG;
@@ -202,34 +207,72 @@ static int make_geom(int q, int nx, int ny, int nz, Geom &g) {
     set_error("grid of %lld nodes exceeds the 32-bit node index of this build", g.N);
     return PRT_E_RANGE;
   }
+  // block shape: (bz, by) threads, npt nodes per thread along y.  Default:
+  // the smallest power of two >= Z (64..256) along z, 256 threads.
   int bz = 64;
   while (bz < g.Z && bz < 256) bz <<= 1;
-  const int by = 256 / bz;
+  int by = 256 / bz;
+  g.npt = 1;
+  const int tz = (flags >> 8) & 15, ty = (flags >> 12) & 15, tn = (flags >> 16) & 3;
+  if (tz) {
+    bz = 1 << tz;
+    by = 1 << ty;
+    g.npt = tn + 1;
+    if (bz < 1 || bz * by > 512 || bz * by < 64 || (bz * by) % 64 || g.npt > 2) {
+      set_error("bad block tuning: %d x %d threads, %d nodes per thread", bz, by, g.npt);
+      return PRT_E_ARG;
+    }
+  }
   g.block = dim3(bz, by, 1);
-  g.grid = dim3((g.Z + bz - 1) / bz, (g.Y + by - 1) / by, g.X);
+  const int rows = (g.Y + g.npt - 1) / g.npt;
+  g.grid = dim3((g.Z + bz - 1) / bz, (rows + by - 1) / by, g.X);
   return PRT_OK;
 }
 
+long long pop_stride(long long n, int elem_bytes) {
+  static long long env_pad = -2;
+  if (env_pad == -2) {
+    const char *e = std::getenv("PRTLB_POP_PAD_BYTES");
+    env_pad = e ? std::atoll(e) : -1;
+  }
+  const long long pad_bytes = env_pad >= 0 ? env_pad : 0;
+  long long pad = (pad_bytes + elem_bytes - 1) / elem_bytes;
+  // keep every array 256-byte aligned
+  long long ld = n + pad;
+  const long long al = 256 / elem_bytes;
+  return (ld + al - 1) / al * al;
+}
+
+static int elem_size(int storage) { return storage == PRT_STORE_FP64 ? 8 : (storage == PRT_STORE_FP32 ? 4 : 2); }
+
 static void fill_args(LbmArgs &a, int q, const Geom &g, const uint8_t *solid, const void *src, void *dst,
-                      double omp, double omm, double force, double s16, double *partials) {
+                      double omp, double omm, double force, double s16, double *partials, int elem_bytes) {
   std::memset(&a, 0, sizeof(a));
   a.solid = solid; a.src = src; a.dst = dst; a.partials = partials;
   a.X = g.X; a.Y = g.Y; a.Z = g.Z; a.N = g.N;
+  a.ld = pop_stride(g.N, elem_bytes);
   a.omp = omp; a.omm = omm; a.g = force; a.half_g = 0.5 * force;
   a.omh = 1 - 0.5 * omm;
+  (void)q;
   a.s16 = (float)s16;
   a.inv_s16 = (float)(1.0 / s16);
 }
 
+template <int Q, int ST, int NPT>
+static void launch_step_n(const Geom &g, const LbmArgs &a, bool skip, bool red, hipStream_t st) {
+  if (skip) {
+    if (red) lbm_step_kernel<Q, ST, true, true, NPT><<<g.grid, g.block, 0, st>>>(a);
+    else lbm_step_kernel<Q, ST, true, false, NPT><<<g.grid, g.block, 0, st>>>(a);
+  } else {
+    if (red) lbm_step_kernel<Q, ST, false, true, NPT><<<g.grid, g.block, 0, st>>>(a);
+    else lbm_step_kernel<Q, ST, false, false, NPT><<<g.grid, g.block, 0, st>>>(a);
+  }
+}
+
 template <int Q, int ST>
 static void launch_step_t(const Geom &g, const LbmArgs &a, bool skip, bool red, hipStream_t st) {
-  if (skip) {
-    if (red) lbm_step_kernel<Q, ST, true, true><<<g.grid, g.block, 0, st>>>(a);
-    else lbm_step_kernel<Q, ST, true, false><<<g.grid, g.block, 0, st>>>(a);
-  } else {
-    if (red) lbm_step_kernel<Q, ST, false, true><<<g.grid, g.block, 0, st>>>(a);
-    else lbm_step_kernel<Q, ST, false, false><<<g.grid, g.block, 0, st>>>(a);
-  }
+  if (g.npt == 2) launch_step_n<Q, ST, 2>(g, a, skip, red, st);
+  else launch_step_n<Q, ST, 1>(g, a, skip, red, st);
 }
 
 static int launch_step(int q, int storage, const Geom &g, const LbmArgs &a, bool skip, bool red,
@@ -265,19 +308,20 @@ static int check_omega(double omp, double omm, double s16, int storage) {
 using namespace prt;
 
 extern "C" int prt_lbm_fill_rest(int q, int storage, int64_t n_nodes, void *f, void *stream) {
-  if (!f || n_nodes < 1 || (q != 9 && q != 19)) {
+  if (!f || n_nodes < 1 || (q != 9 && q != 19) || storage < 0 || storage > 2) {
     set_error("prt_lbm_fill_rest: bad arguments");
     return PRT_E_ARG;
   }
   hipStream_t st = (hipStream_t)stream;
   const int blocks = 2048;
+  const long long ld = pop_stride(n_nodes, elem_size(storage));
   switch (q * 4 + storage) {
-    case 19 * 4 + PRT_STORE_FP64: fill_rest_kernel<19, PRT_STORE_FP64><<<blocks, 256, 0, st>>>(f, n_nodes); break;
-    case 19 * 4 + PRT_STORE_FP32: fill_rest_kernel<19, PRT_STORE_FP32><<<blocks, 256, 0, st>>>(f, n_nodes); break;
-    case 19 * 4 + PRT_STORE_FP16: fill_rest_kernel<19, PRT_STORE_FP16><<<blocks, 256, 0, st>>>(f, n_nodes); break;
-    case 9 * 4 + PRT_STORE_FP64: fill_rest_kernel<9, PRT_STORE_FP64><<<blocks, 256, 0, st>>>(f, n_nodes); break;
-    case 9 * 4 + PRT_STORE_FP32: fill_rest_kernel<9, PRT_STORE_FP32><<<blocks, 256, 0, st>>>(f, n_nodes); break;
-    case 9 * 4 + PRT_STORE_FP16: fill_rest_kernel<9, PRT_STORE_FP16><<<blocks, 256, 0, st>>>(f, n_nodes); break;
+    case 19 * 4 + PRT_STORE_FP64: fill_rest_kernel<19, PRT_STORE_FP64><<<blocks, 256, 0, st>>>(f, ld); break;
+    case 19 * 4 + PRT_STORE_FP32: fill_rest_kernel<19, PRT_STORE_FP32><<<blocks, 256, 0, st>>>(f, ld); break;
+    case 19 * 4 + PRT_STORE_FP16: fill_rest_kernel<19, PRT_STORE_FP16><<<blocks, 256, 0, st>>>(f, ld); break;
+    case 9 * 4 + PRT_STORE_FP64: fill_rest_kernel<9, PRT_STORE_FP64><<<blocks, 256, 0, st>>>(f, ld); break;
+    case 9 * 4 + PRT_STORE_FP32: fill_rest_kernel<9, PRT_STORE_FP32><<<blocks, 256, 0, st>>>(f, ld); break;
+    case 9 * 4 + PRT_STORE_FP16: fill_rest_kernel<9, PRT_STORE_FP16><<<blocks, 256, 0, st>>>(f, ld); break;
     default:
       set_error("unknown storage %d", storage);
       return PRT_E_ARG;
@@ -286,16 +330,26 @@ extern "C" int prt_lbm_fill_rest(int q, int storage, int64_t n_nodes, void *f, v
 }
 
 extern "C" int64_t prt_lbm_partials_count(int q, int nx, int ny, int nz) {
+  // the largest block count any tuning of this grid launches (64 threads)
   Geom g;
   if (make_geom(q, nx, ny, nz, g) != PRT_OK) return -1;
-  return (int64_t)g.grid.x * g.grid.y * g.grid.z;
+  const long long zb = (g.Z + 63) / 64;
+  return (int64_t)(zb * g.Y * g.X);
+}
+
+extern "C" int64_t prt_lbm_pop_stride(int64_t n_nodes, int storage) {
+  if (n_nodes < 1 || storage < 0 || storage > 2) {
+    set_error("prt_lbm_pop_stride: bad arguments");
+    return -1;
+  }
+  return pop_stride(n_nodes, elem_size(storage));
 }
 
 extern "C" int prt_lbm_step(int q, int storage, int nx, int ny, int nz, const uint8_t *solid, const void *f_src,
                             void *f_dst, double omega_p, double omega_m, double force, double fp16_scale,
                             int flags, double *partials, void *stream) {
   Geom g;
-  int rc = make_geom(q, nx, ny, nz, g);
+  int rc = make_geom(q, nx, ny, nz, g, flags);
   if (rc) return rc;
   if ((rc = check_omega(omega_p, omega_m, fp16_scale, storage))) return rc;
   if (!solid || !f_src || !f_dst || f_src == f_dst) {
@@ -303,7 +357,7 @@ extern "C" int prt_lbm_step(int q, int storage, int nx, int ny, int nz, const ui
     return PRT_E_ARG;
   }
   LbmArgs a;
-  fill_args(a, q, g, solid, f_src, f_dst, omega_p, omega_m, force, fp16_scale, partials);
+  fill_args(a, q, g, solid, f_src, f_dst, omega_p, omega_m, force, fp16_scale, partials, elem_size(storage));
   return launch_step(q, storage, g, a, (flags & PRT_LBM_SKIP_SOLID) != 0, partials != nullptr,
                      (hipStream_t)stream);
 }
@@ -312,7 +366,7 @@ extern "C" int prt_lbm_run(int q, int storage, int nx, int ny, int nz, const uin
                            int steps, double omega_p, double omega_m, double force, double fp16_scale, int flags,
                            double *partials, void *stream) {
   Geom g;
-  int rc = make_geom(q, nx, ny, nz, g);
+  int rc = make_geom(q, nx, ny, nz, g, flags);
   if (rc) return rc;
   if ((rc = check_omega(omega_p, omega_m, fp16_scale, storage))) return rc;
   if (!solid || !f_a || !f_b || f_a == f_b || steps < 0) {
@@ -320,7 +374,7 @@ extern "C" int prt_lbm_run(int q, int storage, int nx, int ny, int nz, const uin
     return PRT_E_ARG;
   }
   LbmArgs a;
-  fill_args(a, q, g, solid, f_a, f_b, omega_p, omega_m, force, fp16_scale, nullptr);
+  fill_args(a, q, g, solid, f_a, f_b, omega_p, omega_m, force, fp16_scale, nullptr, elem_size(storage));
   for (int s = 0; s < steps; ++s) {
     a.src = (s & 1) ? f_b : f_a;
     a.dst = (s & 1) ? f_a : f_b;
@@ -357,7 +411,7 @@ extern "C" int prt_lbm_velocity(int q, int storage, int nx, int ny, int nz, cons
   // in the fluid-only mode the populations a solid neighbour would hand
   // back are a fluid node's own two-step-old slots: the other buffer
   fill_args(a, q, g, solid, f_last, const_cast<void *>(f_prev), 1.0, 1.0, force,
-            fp16_scale > 0 ? fp16_scale : 1.0, partials);
+            fp16_scale > 0 ? fp16_scale : 1.0, partials, elem_size(storage));
   hipStream_t st = (hipStream_t)stream;
 #define PRT_VK(QQ, SS)                                                                              \
   do {                                                                                              \

@@ -48,10 +48,19 @@ struct LbmArgs {
   double *partials;
   int X, Y, Z;
   long long N;
+  long long ld;       // element stride between population arrays (>= N)
   double omp, omm, g, half_g;
   double omh;         // 1 - omm/2, formed on the host as the reference does
   float s16, inv_s16; // FP16 storage scale s and 1/s
 };
+
+// Element stride between the population arrays of an n-node grid.  The
+// arrays are padded so that the Q read streams and Q write streams of the
+// pull kernel start at staggered HBM channel offsets instead of the same
+// offset modulo a large power of two (the grids here are often 2^k cubed).
+// PRTLB_POP_PAD_BYTES overrides the pad (for tuning); the rule is the same
+// for every entry point, so buffers sized with prt_lbm_pop_stride() fit.
+long long pop_stride(long long n, int elem_bytes);
 
 // Guo forcing term w_i (1 - omm/2) 3 g e_ix with the reference's association
 // (3D/tools/prtlb_3d.py:439-440); exact constants, contraction off.
@@ -125,9 +134,9 @@ __device__ inline void gather(const LbmArgs &a, const Nbr &nb, int n, R *f) {
     const int s = (i == 0) ? n : nb.at(Lat<Q>::cx[i], Lat<Q>::cy[i], Lat<Q>::cz[i]);
     T v;
     if (SKIP && i != 0 && a.solid[s])
-      v = own[(size_t)Lat<Q>::opp[i] * a.N + n];
+      v = own[(size_t)Lat<Q>::opp[i] * a.ld + n];
     else
-      v = src[(size_t)i * a.N + s];
+      v = src[(size_t)i * a.ld + s];
     if constexpr (ST == PRT_STORE_FP16)
       f[i] = __half2float(v) * ld16<Q>(a, i);
     else
@@ -143,10 +152,10 @@ __device__ inline void scatter(const LbmArgs &a, int n, const R *fp) {
     if constexpr (ST == PRT_STORE_FP16) {
       unsigned short *dst = static_cast<unsigned short *>(a.dst);
       const unsigned short v = __half_as_ushort(__float2half_rn(fp[i] * st16<Q>(a, i)));
-      __builtin_nontemporal_store(v, dst + (size_t)i * a.N + n);
+      __builtin_nontemporal_store(v, dst + (size_t)i * a.ld + n);
     } else {
       T *dst = static_cast<T *>(a.dst);
-      __builtin_nontemporal_store((T)fp[i], dst + (size_t)i * a.N + n);
+      __builtin_nontemporal_store((T)fp[i], dst + (size_t)i * a.ld + n);
     }
   }
 }

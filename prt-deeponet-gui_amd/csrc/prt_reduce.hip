@@ -173,3 +173,30 @@ extern "C" int prt_pairwise_sum_f32(const float *c, int64_t ld, int rows, const 
   pw_combine_kernel<<<rows, 1024, 0, st>>>(lf, rt, lo, (int)n_levels, roots, nchunks, L, stride, scratch, out);
   return launch_status("prt_pairwise_sum_f32 combine launch");
 }
+
+// ------------------------------------------------------------ STREAM copy
+// A plain 16-byte-per-lane grid-stride copy: the measured HBM ceiling the
+// stencil kernels are compared with, and the known-size kernel the HBM
+// traffic counters are calibrated on.
+namespace prt {
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void stream_copy_kernel(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst,
+                                                          long long n16) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+    __builtin_nontemporal_store(src[i], dst + i);
+}
+}  // namespace prt
+
+extern "C" int prt_stream_copy(const void *src, void *dst, int64_t bytes, void *stream) {
+  if (!src || !dst || src == dst || bytes < 16 || (bytes & 15) || ((uintptr_t)src & 15) || ((uintptr_t)dst & 15)) {
+    set_error("prt_stream_copy: need distinct 16-byte aligned buffers and a multiple of 16 bytes");
+    return PRT_E_ARG;
+  }
+  const long long n16 = bytes / 16;
+  long long blocks = (n16 + 255) / 256;
+  if (blocks > 256 * 64) blocks = 256 * 64;
+  stream_copy_kernel<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(static_cast<const u32x4 *>(src),
+                                                                        static_cast<u32x4 *>(dst), n16);
+  return launch_status("prt_stream_copy launch");
+}

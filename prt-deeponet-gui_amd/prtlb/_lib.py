@@ -17,6 +17,11 @@ PRT_STORE = {"fp64": 0, "fp32": 1, "fp16": 2}
 PRT_LBM_SKIP_SOLID = 1
 
 
+def lbm_tune(bz_log2, by_log2, npt):
+    """PRT_LBM_TUNE: launch shape bits for prt_lbm_step / prt_lbm_run."""
+    return (bz_log2 << 8) | (by_log2 << 12) | ((npt - 1) << 16)
+
+
 class PrtError(RuntimeError):
     """A call into libprtlb_hip.so failed."""
 
@@ -80,7 +85,9 @@ _SIGS = {
     "prt_adr_run": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _I, _P]),
     "prt_lbm_batch_desc_size": (_I64, []),
     "prt_adr_batch_desc_size": (_I64, []),
-    "prt_lbm_batch_plan": (_I, [_I, _P, _I, _P, _P, _I64, ctypes.POINTER(_I64)]),
+    "prt_lbm_batch_plan": (_I, [_I, _I, _P, _I, _P, _P, _I64, ctypes.POINTER(_I64)]),
+    "prt_lbm_pop_stride": (_I64, [_I64, _I]),
+    "prt_stream_copy": (_I, [_P, _P, _I64, _P]),
     "prt_adr_batch_plan": (_I, [_P, _I, _P, _P, _I64, ctypes.POINTER(_I64),
                                 ctypes.POINTER(ctypes.c_int)]),
     "prt_lbm_batch_step": (_I, [_I, _I, _P, _P, _I64, _I, _I, _P, _P]),

@@ -96,7 +96,8 @@ class LbmBatch:
         m.force, m.tau = float(force), float(tau_lb)
         m.om_p = 1.0 / m.tau
         m.om_m = trt_minus(m.om_p, magic)
-        m.f = [torch.empty((self.q, m.n), dtype=DTYPES[self.storage], device=self.dev)
+        ld = int(lib().prt_lbm_pop_stride(m.n, self.st))
+        m.f = [torch.empty((self.q, ld), dtype=DTYPES[self.storage], device=self.dev)
                for _ in range(2)]
         s = stream_ptr(self.dev)
         for buf in m.f:
@@ -127,11 +128,11 @@ class LbmBatch:
         dsz = lib().prt_lbm_batch_desc_size()
         desc = np.zeros(dsz * len(mem), np.uint8)
         nb = ctypes.c_int64()
-        call("prt_lbm_batch_plan", self.q, segs, len(mem), desc.ctypes.data, None, 0,
+        call("prt_lbm_batch_plan", self.q, self.st, segs, len(mem), desc.ctypes.data, None, 0,
              ctypes.byref(nb))
         blk = np.zeros(nb.value, np.int32)
-        call("prt_lbm_batch_plan", self.q, segs, len(mem), desc.ctypes.data, blk.ctypes.data,
-             nb.value, ctypes.byref(nb))
+        call("prt_lbm_batch_plan", self.q, self.st, segs, len(mem), desc.ctypes.data,
+             blk.ctypes.data, nb.value, ctypes.byref(nb))
         self.order = mem
         self.desc = torch.from_numpy(desc).to(self.dev)
         self.blk_seg = torch.from_numpy(blk).to(self.dev)

@@ -116,7 +116,9 @@ class StokesSolver:
         self.flags = PRT_LBM_SKIP_SOLID if skip_solid else 0
         self.fp16_scale = float(fp16_scale)
         dt = DTYPES[storage]
-        self.f = [torch.empty((self.q, self.n), dtype=dt, device=self.dev) for _ in range(2)]
+        self.ld = int(lib().prt_lbm_pop_stride(self.n, self.st))
+        self.f = [torch.empty((self.q, self.ld), dtype=dt, device=self.dev) for _ in range(2)]
+        self.tune = 0
         self.latest = 0          # index of the buffer holding the newest state
         self.iters = 0
         npart = lib().prt_lbm_partials_count(self.q, *self.dims)
@@ -136,8 +138,8 @@ class StokesSolver:
             return
         a, b = self.f[self.latest], self.f[1 - self.latest]
         call("prt_lbm_run", self.q, self.st, *self.dims, ptr(self.solid), ptr(a), ptr(b),
-             int(steps), self.om_p, self.om_m, self.force, self.fp16_scale, self.flags,
-             ptr(self.partials) if reduce else None, self._stream())
+             int(steps), self.om_p, self.om_m, self.force, self.fp16_scale,
+             self.flags | self.tune, ptr(self.partials) if reduce else None, self._stream())
         if steps % 2:
             self.latest = 1 - self.latest
         self.iters += steps
@@ -193,7 +195,7 @@ class StokesSolver:
 
     def populations(self):
         """The newest post-collision populations (q, *grid) as stored."""
-        return self.f[self.latest].reshape((self.q,) + self.shape)
+        return self.f[self.latest][:, :self.n].reshape((self.q,) + self.shape)
 
 
 def checked_flow(v, solid, tau_lb, force, name="flow"):
