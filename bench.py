@@ -34,10 +34,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_lbm(g, storage, steps, warmup, skip_solid=False):
+def time_lbm(g, storage, steps, warmup, skip_solid=False, tune=True):
     import torch
     import prtlb
+    from prtlb import autotune
     s = prtlb.StokesSolver(g, force=2e-6, storage=storage, skip_solid=skip_solid)
+    shape = autotune.tune(s, steps=5)["tune"] if tune else 0
     s.run(warmup)
     torch.cuda.synchronize()
     st = torch.cuda.current_stream()
@@ -53,7 +55,7 @@ def time_lbm(g, storage, steps, warmup, skip_solid=False):
     gbs = nodes * bpl / (ms * 1e-3) / 1e9
     out = dict(ms_per_step=ms, mlups=mlups, gbs=gbs, frac=gbs * 1e9 / PEAK_HBM,
                nodes=nodes, fluid_fraction=s.n_fluid / s.n, bytes_per_lu=bpl,
-               mlups_fluid=s.n_fluid / (ms * 1e-3) / 1e6)
+               mlups_fluid=s.n_fluid / (ms * 1e-3) / 1e6, launch_shape=shape)
     del s
     torch.cuda.empty_cache()
     return out
@@ -78,6 +80,29 @@ def time_adr(g, steps, warmup):
     bpv = 2 * 1 * 4 + 3 * 4 + 1          # 21 B per pore voxel-step (ns = 1, 3D, f32)
     return dict(ms_per_step=ms, pore_voxel_steps_per_s=vps, gbs=vps * bpv / 1e9,
                 frac=vps * bpv / PEAK_HBM, bytes_per_voxel=bpv, pore_voxels=T.n_pore)
+
+
+def stream_copy(nbytes=2 * 1024 ** 3, reps=10):
+    """Measured device-copy bandwidth (read + write bytes / time) of each
+    copy variant of the library: the practical HBM ceiling."""
+    import torch
+    from prtlb import _lib
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    s = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for var in (0, 1, 2):
+        for _ in range(2):
+            _lib.call("prt_stream_copy", a.data_ptr(), b.data_ptr(), nbytes, var, s)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            _lib.call("prt_stream_copy", a.data_ptr(), b.data_ptr(), nbytes, var, s)
+        e1.record()
+        e1.synchronize()
+        out["variant%d_GBs" % var] = round(2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
+    out["best_GBs"] = max(out.values())
+    return out
 
 
 def cpu_baseline(budget_s=15.0):
@@ -109,6 +134,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--grid", type=int, default=128, help="edge of the main cube")
     ap.add_argument("--storage", default="fp64", choices=["fp64", "fp32", "fp16"])
+    ap.add_argument("--stream-only", action="store_true",
+                    help="only the device-copy ceiling (counter calibration runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -118,6 +145,9 @@ def main():
     import torch
     import prtlb
     assert torch.cuda.is_available(), "bench needs a HIP device"
+    if args.stream_only:
+        print(json.dumps({"stream_copy": stream_copy()}), flush=True)
+        return
 
     E = args.grid
     log("geometry %d^3 ..." % E)
@@ -143,6 +173,8 @@ def main():
         del g256
         log("transport 128^3 ...")
         extras["adr_128_monod_f32"] = time_adr(g128, args.steps, args.warmup)
+        log("stream copy ...")
+        extras["stream_copy"] = stream_copy()
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")

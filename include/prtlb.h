@@ -244,8 +244,11 @@ int prt_seg_sums(const int32_t *blk0_dev, const int32_t *nblk_dev, int nseg,
 
 /* Plain device-to-device copy of `bytes` (multiple of 16, 16-byte aligned
  * buffers): the STREAM-copy bandwidth ceiling and the known-size kernel for
- * calibrating HBM traffic counters.  No counterpart in the reference. */
-int prt_stream_copy(const void *src, void *dst, int64_t bytes, void *stream);
+ * calibrating HBM traffic counters.  variant 0: 16 B per lane, full grid;
+ * 1: 64 B per lane; 2: grid-stride with nontemporal stores.  No counterpart
+ * in the reference. */
+int prt_stream_copy(const void *src, void *dst, int64_t bytes, int variant,
+                    void *stream);
 
 /* ------------------------------------------------------ NumPy-order sums */
 

@@ -204,9 +204,13 @@ __device__ inline void collide_ref(const LbmArgs &a, bool solid, const double *f
 }
 
 // FP32 collision on deviations g_i = f_i - w_i (the shifted storages).
+// Contraction is off so every kernel that inlines this (single-geometry,
+// batched) evaluates exactly the same operations: a geometry gives the same
+// bits alone or in a batch.
 template <int Q>
 __device__ inline void collide_dev(const LbmArgs &a, bool solid, const float *gq,
                                    float *out, double &ux_out) {
+#pragma clang fp contract(off)
   float dr = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
 #pragma unroll
   for (int i = 0; i < Q; ++i) {

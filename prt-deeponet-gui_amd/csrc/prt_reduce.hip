@@ -175,28 +175,55 @@ extern "C" int prt_pairwise_sum_f32(const float *c, int64_t ld, int rows, const 
 }
 
 // ------------------------------------------------------------ STREAM copy
-// A plain 16-byte-per-lane grid-stride copy: the measured HBM ceiling the
-// stencil kernels are compared with, and the known-size kernel the HBM
-// traffic counters are calibrated on.
+// Plain device copies: the measured HBM ceiling the stencil kernels are
+// compared with, and the known-size kernel the HBM traffic counters are
+// calibrated on.  variant 0: one 16-byte vector per lane over a full grid;
+// 1: four 16-byte vectors per lane; 2: grid-stride loop with nontemporal
+// stores.
 namespace prt {
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-__global__ __launch_bounds__(256) void stream_copy_kernel(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst,
-                                                          long long n16) {
+__global__ __launch_bounds__(256) void copy_flat1(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst,
+                                                  long long n16) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n16) dst[i] = src[i];
+}
+__global__ __launch_bounds__(256) void copy_flat4(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst,
+                                                  long long n16) {
+  const long long base = (long long)blockIdx.x * 1024 + threadIdx.x;
+  u32x4 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (base + k * 256 < n16) v[k] = src[base + k * 256];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (base + k * 256 < n16) dst[base + k * 256] = v[k];
+}
+__global__ __launch_bounds__(256) void copy_stride_nt(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst,
+                                                      long long n16) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
     __builtin_nontemporal_store(src[i], dst + i);
 }
 }  // namespace prt
 
-extern "C" int prt_stream_copy(const void *src, void *dst, int64_t bytes, void *stream) {
-  if (!src || !dst || src == dst || bytes < 16 || (bytes & 15) || ((uintptr_t)src & 15) || ((uintptr_t)dst & 15)) {
-    set_error("prt_stream_copy: need distinct 16-byte aligned buffers and a multiple of 16 bytes");
+extern "C" int prt_stream_copy(const void *src, void *dst, int64_t bytes, int variant, void *stream) {
+  if (!src || !dst || src == dst || bytes < 16 || (bytes & 15) || ((uintptr_t)src & 15) || ((uintptr_t)dst & 15) ||
+      variant < 0 || variant > 2) {
+    set_error("prt_stream_copy: need distinct 16-byte aligned buffers, a multiple of 16 bytes, variant 0..2");
     return PRT_E_ARG;
   }
   const long long n16 = bytes / 16;
-  long long blocks = (n16 + 255) / 256;
-  if (blocks > 256 * 64) blocks = 256 * 64;
-  stream_copy_kernel<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(static_cast<const u32x4 *>(src),
-                                                                        static_cast<u32x4 *>(dst), n16);
+  const u32x4 *s = static_cast<const u32x4 *>(src);
+  u32x4 *d = static_cast<u32x4 *>(dst);
+  hipStream_t st = (hipStream_t)stream;
+  if (variant == 0) {
+    copy_flat1<<<(unsigned)((n16 + 255) / 256), 256, 0, st>>>(s, d, n16);
+  } else if (variant == 1) {
+    copy_flat4<<<(unsigned)((n16 + 1023) / 1024), 256, 0, st>>>(s, d, n16);
+  } else {
+    long long blocks = (n16 + 255) / 256;
+    if (blocks > 256 * 64) blocks = 256 * 64;
+    copy_stride_nt<<<(unsigned)blocks, 256, 0, st>>>(s, d, n16);
+  }
   return launch_status("prt_stream_copy launch");
 }

@@ -87,7 +87,7 @@ _SIGS = {
     "prt_adr_batch_desc_size": (_I64, []),
     "prt_lbm_batch_plan": (_I, [_I, _I, _P, _I, _P, _P, _I64, ctypes.POINTER(_I64)]),
     "prt_lbm_pop_stride": (_I64, [_I64, _I]),
-    "prt_stream_copy": (_I, [_P, _P, _I64, _P]),
+    "prt_stream_copy": (_I, [_P, _P, _I64, _I, _P]),
     "prt_adr_batch_plan": (_I, [_P, _I, _P, _P, _I64, ctypes.POINTER(_I64),
                                 ctypes.POINTER(ctypes.c_int)]),
     "prt_lbm_batch_step": (_I, [_I, _I, _P, _P, _I64, _I, _I, _P, _P]),

@@ -78,10 +78,31 @@ def campaign_specs(n_geom=64, edge=(48, 160), radius=(3.0, 8.0), porosity=(0.25,
     return specs
 
 
-def make_geometry(spec):
-    if len(spec["shape"]) == 2:
-        return disk_pack(spec["shape"], spec["radius"], spec["porosity"], seed=spec["geom_seed"])
-    return sphere_pack(spec["shape"], spec["radius"], spec["porosity"], seed=spec["geom_seed"])
+def percolates(g):
+    """True when the pore space connects the x = 0 face to the x = nx-1
+    face (face connectivity), i.e. a body force along x can drive flow."""
+    from scipy import ndimage
+    pore = np.asarray(g) == PORE
+    lab, _ = ndimage.label(pore)
+    a = set(np.unique(lab[0][pore[0]]).tolist()) - {0}
+    b = set(np.unique(lab[-1][pore[-1]]).tolist()) - {0}
+    return bool(a & b)
+
+
+def make_geometry(spec, max_tries=50):
+    """The spec's pack, redrawn with the next seed until it percolates along
+    x (the reference's builders likewise keep only spanning rocks,
+    3D/tools/build_dataset_2d.py:285 keep_spanning_cluster / percolates)."""
+    seed = spec["geom_seed"]
+    for _ in range(max_tries):
+        if len(spec["shape"]) == 2:
+            g = disk_pack(spec["shape"], spec["radius"], spec["porosity"], seed=seed)
+        else:
+            g = sphere_pack(spec["shape"], spec["radius"], spec["porosity"], seed=seed)
+        if percolates(g):
+            return g
+        seed += 1
+    raise RuntimeError("no percolating pack for %r in %d draws" % (spec, max_tries))
 
 
 def _adr_kw(spec, kinetics):

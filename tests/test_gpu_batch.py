@@ -92,11 +92,11 @@ def test_server_result_same_alone_and_in_mix():
 
 
 def test_campaign_batched_matches_sequential_fp32():
-    specs = C.campaign_specs(n_geom=3, edge=(20, 28), seed=5)
+    specs = C.campaign_specs(n_geom=3, edge=(32, 40), radius=(2.0, 4.0), seed=5)
     geoms = {s["key"]: C.make_geometry(s) for s in specs}
     rb, _ = C.run_batched(specs, geoms, storage="fp32", adr_steps=200)
     rs, _ = C.run_sequential(specs, geoms, storage="fp32", adr_steps=200)
     for k in rb:
-        assert rb[k]["flow_iters"] == rs[k]["flow_iters"]
+        assert rb[k]["flow_iters"] == rs[k]["flow_iters"] < 30000
         assert rb[k]["permeability"] == pytest.approx(rs[k]["permeability"], rel=1e-12)
         assert np.array_equal(rb[k]["out_last"], rs[k]["out_last"])
