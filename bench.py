@@ -82,6 +82,16 @@ def time_adr(g, steps, warmup):
                 frac=vps * bpv / PEAK_HBM, bytes_per_voxel=bpv, pore_voxels=T.n_pore)
 
 
+def measured_traffic(key):
+    """HBM bytes per launch of this workload from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (profiles/), or None if not measured."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        return json.load(open(path))[key]["traffic_bytes"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def stream_copy(nbytes=2 * 1024 ** 3, reps=10):
     """Measured device-copy bandwidth (read + write bytes / time) of each
     copy variant of the library: the practical HBM ceiling."""
@@ -199,7 +209,8 @@ def main():
                    "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "achieved": round(main_r["gbs"], 1),
                      "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                     "frac": round(main_r["frac"], 4), "traffic": None},
+                     "frac": round(main_r["frac"], 4),
+                     "traffic": measured_traffic("d3q19_%s_%d" % (args.storage, E))},
         "cpu_baseline": cpu,
         "mlups_fluid": round(main_r["mlups_fluid"], 1),
         "extras": extras,

@@ -144,6 +144,34 @@ class StokesSolver:
             self.latest = 1 - self.latest
         self.iters += steps
 
+    def _launch(self, steps, reduce):
+        a, b = self.f[self.latest], self.f[1 - self.latest]
+        call("prt_lbm_run", self.q, self.st, *self.dims, ptr(self.solid), ptr(a), ptr(b),
+             int(steps), self.om_p, self.om_m, self.force, self.fp16_scale,
+             self.flags | self.tune, ptr(self.partials) if reduce else None, self._stream())
+
+    def tick_graph(self, every=50):
+        """`every` iterations (even) ending in the |u_x| reduction and its
+        device-side sum, replayed from a captured HIP graph: one host launch
+        per tick instead of every + 1.  Returns nothing; read mean_abs_ux_ready()."""
+        if every % 2:
+            raise ValueError("a captured tick must have an even length")
+        key = (every, self.latest, self.tune)
+        if getattr(self, "_graph_key", None) != key:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._launch(every, True)
+                call("prt_sum_f64", ptr(self.partials), self.partials.numel(), ptr(self.scalar),
+                     self._stream())
+            self._graph, self._graph_key = g, key
+        self._graph.replay()
+        self.iters += every
+
+    def mean_abs_ux_ready(self):
+        """Mean |u_x| over fluid nodes from the sum a captured tick left."""
+        s = float(self.scalar[0].item())
+        return s / self.n_fluid if self.n_fluid else 0.0
+
     def mean_abs_ux(self):
         """Mean |u_x| over fluid nodes from the last reducing iteration."""
         call("prt_sum_f64", ptr(self.partials), self.partials.numel(), ptr(self.scalar),
@@ -151,10 +179,15 @@ class StokesSolver:
         s = float(self.scalar[0].item())
         return s / self.n_fluid if self.n_fluid else 0.0
 
-    def solve(self, nit=30000, tol=1e-6, info=None, every=50, name=None):
-        """Iterate until FlowConvergence says stop or nit iterations ran."""
+    def solve(self, nit=30000, tol=1e-6, info=None, every=50, name=None, graph=None):
+        """Iterate until FlowConvergence says stop or nit iterations ran.
+
+        graph: replay each convergence interval from a captured HIP graph
+        (default: for grids under 2^20 nodes, where launches dominate)."""
         conv = FlowConvergence(tol=tol, every=every, max_it=nit,
                                name=name or "%dD flow" % self.nd)
+        if graph is None:
+            graph = self.n < (1 << 20) and conv.every % 2 == 0
         it = 0
         nit = int(nit)
         if conv.tol <= 0:
@@ -164,6 +197,13 @@ class StokesSolver:
             while it < nit:
                 chunk = min(conv.every - (it % conv.every), nit - it)
                 at_check = (it + chunk) % conv.every == 0
+                if graph and at_check and chunk == conv.every:
+                    self.tick_graph(chunk)
+                    it += chunk
+                    um = self.mean_abs_ux_ready() if self.n_fluid else 0.0
+                    if conv.done(it - 1, um):
+                        break
+                    continue
                 self.run(chunk, reduce=at_check)
                 it += chunk
                 if at_check:
@@ -232,7 +272,7 @@ def checked_flow(v, solid, tau_lb, force, name="flow"):
 
 
 def _stokes(g, nd, nit, force, tau_lb, magic, tol, info, storage, skip_solid,
-            return_f64, as_tensor, device):
+            return_f64, as_tensor, device, graph=None):
     if isinstance(g, torch.Tensor):
         if g.dim() != nd:
             raise ValueError("stokes_d%s needs a %dD grid" % ("2q9" if nd == 2 else "3q19", nd))
@@ -240,7 +280,7 @@ def _stokes(g, nd, nit, force, tau_lb, magic, tol, info, storage, skip_solid,
         raise ValueError("stokes_d%s needs a %dD grid" % ("2q9" if nd == 2 else "3q19", nd))
     s = StokesSolver(g, force=force, tau_lb=tau_lb, magic=magic, storage=storage,
                      skip_solid=skip_solid, device=device)
-    s.solve(nit=nit, tol=tol, info=info, name="%dD flow" % nd)
+    s.solve(nit=nit, tol=tol, info=info, name="%dD flow" % nd, graph=graph)
     v32, v64, sux = s.velocity(f64=return_f64)
     solid = s.solid.reshape(s.shape).bool()
     checked_flow(v32, solid, tau_lb, force)
@@ -259,7 +299,7 @@ def _stokes(g, nd, nit, force, tau_lb, magic, tol, info, storage, skip_solid,
 
 def stokes_d2q9(g, nit=30000, force=2e-6, tau_lb=0.8, magic=3.0 / 16.0, tol=1e-6,
                 info=None, *, storage="fp64", skip_solid=False, return_f64=False,
-                as_tensor=False, device=None):
+                as_tensor=False, device=None, graph=None):
     """D2Q9 TRT Stokes flow on the GPU; drop-in for 3D/tools/prtlb_2d.py:348.
 
     Returns v float32 (3, nx, ny) with v[2] = 0 and zero in solid.  Extra
@@ -268,18 +308,18 @@ def stokes_d2q9(g, nit=30000, force=2e-6, tau_lb=0.8, magic=3.0 / 16.0, tol=1e-6
     tensors instead of NumPy).  info gains permeability, porosity,
     mean_ux_pore and storage."""
     return _stokes(g, 2, nit, force, tau_lb, magic, tol, info, storage, skip_solid,
-                   return_f64, as_tensor, device)
+                   return_f64, as_tensor, device, graph)
 
 
 def stokes_d3q19(g, nit=30000, force=2e-6, tau_lb=0.8, magic=3.0 / 16.0, tol=1e-6,
                  info=None, *, storage="fp64", skip_solid=False, return_f64=False,
-                 as_tensor=False, device=None):
+                 as_tensor=False, device=None, graph=None):
     """D3Q19 TRT Stokes flow on the GPU; drop-in for 3D/tools/prtlb_3d.py:348.
 
     Returns v float32 (3, nx, ny, nz), zero in solid.  Same extra options as
     stokes_d2q9."""
     return _stokes(g, 3, nit, force, tau_lb, magic, tol, info, storage, skip_solid,
-                   return_f64, as_tensor, device)
+                   return_f64, as_tensor, device, graph)
 
 
 def plane_flux(v64):

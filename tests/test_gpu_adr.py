@@ -154,3 +154,26 @@ def test_sealed_half_and_larger_parity():
     outo, _ = O.solve_adr(gs, vs, chem=O.Chemistry(["C"], [1.0], [0.0], [1.0], [False],
                                                    ks_donor=0.5), **kw)
     assert np.array_equal(out, outo)
+
+
+def test_graph_replays_match_eager():
+    g = prtlb.disk_pack((128, 128), 6.0, 0.6, seed=1)
+    v = prtlb.stokes_d2q9(g, nit=3000, force=1e-5, tol=1e-6)
+    T1 = prtlb.TransportSolver(g, v, 10.0, 0.0, 1, chem=prtlb.chemistry.first_order(True),
+                               da_abio=1.0)
+    T2 = prtlb.TransportSolver(g, v, 10.0, 0.0, 1, chem=prtlb.chemistry.first_order(True),
+                               da_abio=1.0)
+    T1.use_graph, T2.use_graph = True, False
+    for k in range(5):
+        T1.step_n(T1.GRAPH_STEPS)
+        T2.step_n(T2.GRAPH_STEPS)
+        import torch
+        torch.cuda.synchronize()
+        assert torch.equal(T1.current(), T2.current()), k
+    out1, _ = prtlb.solve_adr(g, v, 10.0, 0.0, 4, n_species=1, steps=1000,
+                              chem=prtlb.chemistry.first_order(True), da_abio=1.0)
+    outo, _ = O.solve_adr(g, v, 10.0, 0.0, 4, n_species=1, steps=1000,
+                          chem=O.Chemistry(["C"], [1.0], [0.0], [1.0], [False], biotic_on=False,
+                                           abiotic_index=0, abiotic_surface_only=True),
+                          da_abio=1.0)
+    assert np.array_equal(out1, outo)

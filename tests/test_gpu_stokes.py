@@ -158,3 +158,24 @@ def test_mach_warning_and_refusals():
         prtlb.stokes_d2q9(np.full((8, 8), 2, np.uint8), tau_lb=0.5)
     with pytest.raises(ValueError):
         prtlb.stokes_d3q19(np.full((8, 8), 2, np.uint8))
+
+
+def test_graph_replays_match_eager():
+    # every replay of a captured tick gives the eager result, with eager work
+    # and synchronisations between replays
+    g = prtlb.disk_pack((128, 128), 6.0, 0.6, seed=0)
+    eager = prtlb.StokesSolver(g, force=2e-6)
+    graphed = prtlb.StokesSolver(g, force=2e-6)
+    for k in range(4):
+        eager.run(50, reduce=True)
+        ue = eager.mean_abs_ux()
+        graphed.tick_graph(50)
+        ug = graphed.mean_abs_ux_ready()
+        torch.cuda.synchronize()
+        assert ug == ue, k
+        assert torch.equal(eager.populations(), graphed.populations()), k
+        _ = torch.ones(1000, device="cuda").sum().item()     # unrelated eager work
+    i1, i2 = {}, {}
+    v1 = prtlb.stokes_d2q9(g, nit=5000, tol=1e-8, info=i1, graph=True)
+    v2 = prtlb.stokes_d2q9(g, nit=5000, tol=1e-8, info=i2, graph=False)
+    assert np.array_equal(v1, v2) and i1["flow_iters"] == i2["flow_iters"]
