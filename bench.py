@@ -135,6 +135,106 @@ def cpu_baseline(budget_s=15.0):
                        "(%.1f s) on %s" % (nit, dt, platform.processor() or platform.machine()))
 
 
+def precision_study(cases=((256, 0.25), (256, 0.45), (384, 0.25), (384, 0.45)),
+                    nit=20000, tol=1e-8, adr_steps=2000):
+    """Storage-precision errors against the FP64 GPU solve on seeded sphere
+    packs: velocity (max |dv| / max |v|), Darcy permeability (relative) and,
+    after adr_steps of Monod transport (Pe 10, Da 1, Ks 0.5), concentration
+    (max |dC|) against the FP64 transport on the FP64 flow."""
+    import numpy as np
+    import torch
+    import prtlb
+    out = []
+    for E, phi in cases:
+        g = prtlb.sphere_pack((E, E, E), (4.0, 8.0), phi, seed=0)
+        ref = {}
+        res = {"grid": E, "porosity_target": phi, "porosity": float((g == prtlb.PORE).mean())}
+        for st in ("fp64", "fp32", "fp16"):
+            S = prtlb.StokesSolver(g, force=2e-6, storage=st)
+            t0 = time.perf_counter()
+            conv = S.solve(nit=nit, tol=tol)
+            torch.cuda.synchronize()
+            v32, v64, sux = S.velocity(f64=True)
+            k = S.permeability(sux)
+            r = {"iters": conv.iters, "converged": bool(conv.converged),
+                 "solve_s": round(time.perf_counter() - t0, 2), "permeability": k}
+            v64 = v64.cpu().numpy()
+            del S, v32
+            torch.cuda.empty_cache()
+            if st == "fp64":
+                ref = {"v": v64, "k": k}
+            else:
+                r["vel_err"] = float(np.abs(v64 - ref["v"]).max() / np.abs(ref["v"]).max())
+                r["perm_rel_err"] = abs(k - ref["k"]) / abs(ref["k"])
+            vel = v64.astype(np.float32)
+            prec = "fp64" if st == "fp64" else "fp32"
+            T = prtlb.TransportSolver(g, v64 if st == "fp64" else vel, 10.0, 1.0, 1,
+                                      chem=prtlb.chemistry.monod(0.5), precision=prec)
+            T.step_n(adr_steps)
+            c = T.current().double().cpu().numpy()
+            del T
+            torch.cuda.empty_cache()
+            if st == "fp64":
+                ref["c"] = c
+            else:
+                r["conc_err"] = float(np.abs(c - ref["c"]).max())
+            res[st] = r
+            log("precision", E, phi, st, {kk: vv for kk, vv in r.items()})
+        out.append(res)
+    return out
+
+
+def campaign_study(n_geom=64, edge=(48, 160), seed=0, adr_steps=None, max_steps=100000):
+    """The batched campaign: n_geom seeded sphere packs (edges U{48..160},
+    radii U(3,8), porosity U(0.25,0.5)), Pe log-U(0.1,100), Da log-U(0.01,10),
+    irreversible sorption.  Runs it twice: FP32, batched (one launch per
+    iteration for all rocks, then for all transport solves) and FP64 one at a
+    time (the reference's schedule).  Reports converged solves per hour of
+    each and the permeability error of the batched FP32 result."""
+    import numpy as np
+    from prtlb import campaign as C
+    specs = C.campaign_specs(n_geom=n_geom, edge=edge, seed=seed)
+    t0 = time.perf_counter()
+    geoms = {s["key"]: C.make_geometry(s) for s in specs}
+    log("campaign geometry %.1f s" % (time.perf_counter() - t0))
+    rb, tb = C.run_batched(specs, geoms, storage="fp32", adr_steps=adr_steps, max_steps=max_steps)
+    log("campaign batched fp32 %.1f s" % tb)
+    rs, ts = C.run_sequential(specs, geoms, storage="fp64", adr_steps=adr_steps,
+                              max_steps=max_steps)
+    log("campaign sequential fp64 %.1f s" % ts)
+    perr = [abs(rb[k]["permeability"] - rs[k]["permeability"]) / abs(rs[k]["permeability"])
+            for k in rb]
+    nodes = sum(int(np.prod(s["shape"])) for s in specs)
+    return {"n_solves": len(specs), "box_nodes": nodes,
+            "batched_fp32_s": round(tb, 2), "sequential_fp64_s": round(ts, 2),
+            "batched_solves_per_hour": len(specs) / tb * 3600,
+            "sequential_solves_per_hour": len(specs) / ts * 3600,
+            "speedup": ts / tb, "max_perm_rel_err": max(perr),
+            "median_perm_rel_err": float(np.median(perr)),
+            "flow_iters_capped": sum(1 for k in rs if rs[k]["flow_iters"] >= 30000),
+            "adr_steps_mean": float(np.mean([rs[k]["adr_steps"] for k in rs]))}
+
+
+def serve_study(n_req=48, rate=4.0, seed=0, budget=200e9, max_steps=100000):
+    """Serving: Poisson arrivals (rate per s) of solve requests, 30% D2Q9
+    (256^2..1024^2 disk packs), 70% D3Q19 (64^3..256^3 sphere packs), Pe and
+    Da log-uniform, reversible sorption; HBM budget admission; continuous
+    batching.  Reports p50/p99 time from arrival to converged field and
+    solves per hour."""
+    from prtlb import serve
+    t0 = time.perf_counter()
+    reqs = serve.poisson_requests(n_req, rate=rate, seed=seed)
+    log("serve geometry %.1f s" % (time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    done = serve.Server(hbm_budget=budget, max_steps=max_steps).run(reqs, realtime=True)
+    wall = time.perf_counter() - t0
+    out = serve.latency_summary(done, wall)
+    out.update(rate_per_s=rate, hbm_budget_GB=budget / 1e9,
+               n_2d=sum(1 for r in done if r.g.ndim == 2),
+               adr_steps_mean=float(sum(r.result["info"]["steps"] for r in done) / len(done)))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +246,13 @@ def main():
     ap.add_argument("--storage", default="fp64", choices=["fp64", "fp32", "fp16"])
     ap.add_argument("--stream-only", action="store_true",
                     help="only the device-copy ceiling (counter calibration runs)")
+    ap.add_argument("--precision", action="store_true",
+                    help="storage-precision error study (256^3, 384^3)")
+    ap.add_argument("--campaign", type=int, default=0,
+                    help="batched-campaign study with this many rocks")
+    ap.add_argument("--serve", type=int, default=0,
+                    help="serving study with this many Poisson requests")
+    ap.add_argument("--adr-steps", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -157,6 +264,16 @@ def main():
     assert torch.cuda.is_available(), "bench needs a HIP device"
     if args.stream_only:
         print(json.dumps({"stream_copy": stream_copy()}), flush=True)
+        return
+    if args.precision:
+        print(json.dumps({"precision": precision_study()}), flush=True)
+        return
+    if args.campaign:
+        print(json.dumps({"campaign": campaign_study(args.campaign, adr_steps=args.adr_steps)}),
+              flush=True)
+        return
+    if args.serve:
+        print(json.dumps({"serve": serve_study(args.serve)}), flush=True)
         return
 
     E = args.grid
