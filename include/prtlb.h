@@ -240,6 +240,46 @@ int prt_seg_sums(const int32_t *blk0_dev, const int32_t *nblk_dev, int nseg,
                  int width, const double *partials, double *out,
                  void *stream);
 
+/* ------------------------------------------------------------- geometry */
+
+/* Seeded grain packs on the GPU.  The host draws the candidate grains
+ * (centres (n_grains, nd) and radii, in generator order) and bins them into
+ * cubic cells of edge cell_edge (CSR: cell_start[ncells + 1], cell_items,
+ * ascending grain order per cell).  first[n] = index of the first candidate
+ * whose periodic minimum-image distance to voxel n is <= its radius, or
+ * n_grains.  Grid (nx, ny, nz), nz = 1 for 2D.  Build-defined workload
+ * generator (the reference's generators threshold Gaussian fields). */
+int prt_grain_first_cover(int nx, int ny, int nz, const double *centres,
+                          const double *radii, int n_grains,
+                          const int32_t *cell_start, const int32_t *cell_items,
+                          int cell_edge, int32_t *first, void *stream);
+
+/* hist[k] += number of voxels whose first cover is k (hist zeroed by the
+ * caller, n_grains + 1 entries). */
+int prt_grain_histogram(const int32_t *first, int64_t n_voxels, int32_t *hist,
+                        void *stream);
+
+/* g[n] = first[n] < n_grains_used ? solid_code : pore_code. */
+int prt_grain_mask(const int32_t *first, int64_t n_voxels, int n_grains_used,
+                   int solid_code, int pore_code, uint8_t *g, void *stream);
+
+/* Pore voxels not face-connected (no wrap) to a pore voxel on x = 0 become
+ * solid_code, in place; *removed_host = their count.  reach: n bytes,
+ * scratch2: 2 ints (device).  Synchronises the stream.  Replaces
+ * keep_inlet_connected, 3D/tools/prtlb_3d.py:530-564. */
+int prt_keep_inlet_connected(uint8_t *g, int nx, int ny, int nz, int pore_code,
+                             int solid_code, uint8_t *reach, int *scratch2,
+                             int max_sweeps, int *removed_host, void *stream);
+
+/* Pore voxels within `layers` face dilations (no wrap) of the non-pore get
+ * `code`, in place; *marked_host = their count.  work_a, work_b: n bytes;
+ * scratch1: 1 int.  Synchronises the stream.  Replaces coat_with_biofilm,
+ * 3D/tools/prtlb_3d.py:567-593. */
+int prt_coat_with_biofilm(uint8_t *g, int nx, int ny, int nz, int pore_code,
+                          int code, int layers, uint8_t *work_a,
+                          uint8_t *work_b, int *scratch1, int *marked_host,
+                          void *stream);
+
 /* --------------------------------------------------------- measurement */
 
 /* Plain device-to-device copy of `bytes` (multiple of 16, 16-byte aligned

@@ -71,20 +71,11 @@ __device__ inline bool adr_voxel(const AdrArgs<R> &a, int x, int y, int z, R *cn
   }
   const int bf_self = (fl >> 1) & 1;
 
-  // the inlet pin (3D/tools/prtlb_3d.py:1398-1400) applied as the value is
-  // read: a voxel on x = 0 that is read at all is a pore voxel
-  auto ldc = [&](int j, int idx, int xc) -> R {
-    return (xc == 0 && a.pinned[j]) ? a.fed[j] : a.cin[(size_t)j * a.N + idx];
-  };
-
-  R c[NSM], tr[NSM];
-#pragma unroll
-  for (int j = 0; j < NSM; ++j) {
-    c[j] = j < ns ? ldc(j, n, x) : R(0);
-    tr[j] = R(0);
-  }
-
-  R divu = R(0);
+  // Every neighbour value the stencil may need is loaded up front and
+  // unconditionally (indices are clamped, so always in bounds); the face
+  // rules then pick among loaded values.  Loads that depended on neighbour
+  // flags would chain several memory round trips per axis.
+  int nidx[ND][4], nx0[ND][4];           // [axis][m2, m1, p1, p2]
 #pragma unroll
   for (int ax = 0; ax < ND; ++ax) {
     // axis geometry: 3D axes x, y, z; a 2D grid is (X, 1, Z) with axes x, z
@@ -94,28 +85,66 @@ __device__ inline bool adr_voxel(const AdrArgs<R> &a, int x, int y, int z, R *cn
     const int p = ax3 == 0 ? x : (ax3 == 1 ? y : z);
     const int pm1 = p > 0 ? p - 1 : 0, pm2 = p > 1 ? p - 2 : 0;
     const int pp1 = p < ext - 1 ? p + 1 : ext - 1, pp2 = p < ext - 2 ? p + 2 : ext - 1;
-    const int im1 = n + (pm1 - p) * stride, im2 = n + (pm2 - p) * stride;
-    const int ip1 = n + (pp1 - p) * stride, ip2 = n + (pp2 - p) * stride;
-    const int xm1 = ax3 == 0 ? pm1 : x, xm2 = ax3 == 0 ? pm2 : x;
-    const int xp1 = ax3 == 0 ? pp1 : x, xp2 = ax3 == 0 ? pp2 : x;
-    const uint8_t fm1 = a.flags[im1], fp1 = a.flags[ip1];
-    const bool o_lo = fm1 & 1, o_hi = fp1 & 1;
-    const bool f_lo = o_lo && (a.flags[im2] & 1);
-    const bool f_hi = o_hi && (a.flags[ip2] & 1);
+    nidx[ax][0] = n + (pm2 - p) * stride;
+    nidx[ax][1] = n + (pm1 - p) * stride;
+    nidx[ax][2] = n + (pp1 - p) * stride;
+    nidx[ax][3] = n + (pp2 - p) * stride;
+    nx0[ax][0] = ax3 == 0 ? pm2 : x;
+    nx0[ax][1] = ax3 == 0 ? pm1 : x;
+    nx0[ax][2] = ax3 == 0 ? pp1 : x;
+    nx0[ax][3] = ax3 == 0 ? pp2 : x;
+  }
+  uint8_t nf[ND][4];
+  R um[ND], uc[ND], up[ND];
+#pragma unroll
+  for (int ax = 0; ax < ND; ++ax) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) nf[ax][k] = a.flags[nidx[ax][k]];
     const R *ua = a.u + (size_t)ax * a.N;
-    const R uc = ua[n];
-    const R u_lo = o_lo ? R(0.5) * (uc + ua[im1]) : R(0);
-    const R u_hi = o_hi ? R(0.5) * (uc + ua[ip1]) : R(0);
+    um[ax] = ua[nidx[ax][1]];
+    uc[ax] = ua[n];
+    up[ax] = ua[nidx[ax][2]];
+  }
+  // the inlet pin (3D/tools/prtlb_3d.py:1398-1400) applied as the value is
+  // read: a voxel on x = 0 that is read at all is a pore voxel
+  R c[NSM], tr[NSM], cn[NSM][ND][4];
+#pragma unroll
+  for (int j = 0; j < NSM; ++j) {
+    tr[j] = R(0);
+    c[j] = R(0);
+    if (j >= ns) continue;
+    const R *cj = a.cin + (size_t)j * a.N;
+    const R v = cj[n];
+    c[j] = (x == 0 && a.pinned[j]) ? a.fed[j] : v;
+    if (!a.mobile[j]) continue;
+#pragma unroll
+    for (int ax = 0; ax < ND; ++ax)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const R w = cj[nidx[ax][k]];
+        cn[j][ax][k] = (nx0[ax][k] == 0 && a.pinned[j]) ? a.fed[j] : w;
+      }
+  }
+
+  R divu = R(0);
+#pragma unroll
+  for (int ax = 0; ax < ND; ++ax) {
+    const uint8_t fm1 = nf[ax][1], fp1 = nf[ax][2];
+    const bool o_lo = fm1 & 1, o_hi = fp1 & 1;
+    const bool f_lo = o_lo && (nf[ax][0] & 1);
+    const bool f_hi = o_hi && (nf[ax][3] & 1);
+    const R u_lo = o_lo ? R(0.5) * (uc[ax] + um[ax]) : R(0);
+    const R u_hi = o_hi ? R(0.5) * (uc[ax] + up[ax]) : R(0);
     divu = (ax == 0) ? R(0) + (u_hi - u_lo) : divu + (u_hi - u_lo);
     const int bm1 = (fm1 >> 1) & 1, bp1 = (fp1 >> 1) & 1;
 #pragma unroll
     for (int j = 0; j < NSM; ++j) {
       if (j >= ns || !a.mobile[j]) continue;
       const R C = c[j];
-      const R Cm1 = o_lo ? ldc(j, im1, xm1) : C;
-      const R Cp1 = o_hi ? ldc(j, ip1, xp1) : C;
-      const R Cm2 = f_lo ? ldc(j, im2, xm2) : Cm1;
-      const R Cp2 = f_hi ? ldc(j, ip2, xp2) : Cp1;
+      const R Cm1 = o_lo ? cn[j][ax][1] : C;
+      const R Cp1 = o_hi ? cn[j][ax][2] : C;
+      const R Cm2 = f_lo ? cn[j][ax][0] : Cm1;
+      const R Cp2 = f_hi ? cn[j][ax][3] : Cp1;
       // 2x2 table picked with selects: a runtime index into the kernel
       // argument block would copy the whole block to scratch memory
       const R t0 = bf_self ? a.dface[j][2] : a.dface[j][0];
@@ -161,16 +190,7 @@ __device__ inline bool adr_voxel(const AdrArgs<R> &a, int x, int y, int z, R *cn
       // closed faces over 2*nd, edge-replicated neighbours (:1000-1018)
       int closed = 0;
 #pragma unroll
-      for (int ax = 0; ax < ND; ++ax) {
-        const int ax3 = (ND == 2 && ax == 1) ? 2 : ax;
-        const int stride = ax3 == 0 ? a.Y * a.Z : (ax3 == 1 ? a.Z : 1);
-        const int ext = ax3 == 0 ? a.X : (ax3 == 1 ? a.Y : a.Z);
-        const int p = ax3 == 0 ? x : (ax3 == 1 ? y : z);
-        const int im = n + ((p > 0 ? p - 1 : 0) - p) * stride;
-        const int ip = n + ((p < ext - 1 ? p + 1 : ext - 1) - p) * stride;
-        closed += !(a.flags[im] & 1);
-        closed += !(a.flags[ip] & 1);
-      }
+      for (int ax = 0; ax < ND; ++ax) closed += !(nf[ax][1] & 1) + !(nf[ax][2] & 1);
       surf = R(closed) / R(2 * ND);
       Ra = (a.ka * cj) * surf;
     } else {
@@ -206,16 +226,7 @@ __device__ inline bool adr_voxel(const AdrArgs<R> &a, int x, int y, int z, R *cn
     if (a.surface_only) {
       int closed = 0;
 #pragma unroll
-      for (int ax = 0; ax < ND; ++ax) {
-        const int ax3 = (ND == 2 && ax == 1) ? 2 : ax;
-        const int stride = ax3 == 0 ? a.Y * a.Z : (ax3 == 1 ? a.Z : 1);
-        const int ext = ax3 == 0 ? a.X : (ax3 == 1 ? a.Y : a.Z);
-        const int p = ax3 == 0 ? x : (ax3 == 1 ? y : z);
-        const int im = n + ((p > 0 ? p - 1 : 0) - p) * stride;
-        const int ip = n + ((p < ext - 1 ? p + 1 : ext - 1) - p) * stride;
-        closed += !(a.flags[im] & 1);
-        closed += !(a.flags[ip] & 1);
-      }
+      for (int ax = 0; ax < ND; ++ax) closed += !(nf[ax][1] & 1) + !(nf[ax][2] & 1);
       surf = R(closed) / R(2 * ND);
     }
     R fw = (a.k_sorb * cs) * surf;

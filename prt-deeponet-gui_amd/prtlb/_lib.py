@@ -93,6 +93,13 @@ _SIGS = {
     "prt_lbm_batch_step": (_I, [_I, _I, _P, _P, _I64, _I, _I, _P, _P]),
     "prt_adr_batch_step": (_I, [_P, _P, _I64, _I, _I, _P, _P]),
     "prt_seg_sums": (_I, [_P, _P, _I, _I, _P, _P, _P]),
+    "prt_grain_first_cover": (_I, [_I, _I, _I, _P, _P, _I, _P, _P, _I, _P, _P]),
+    "prt_grain_histogram": (_I, [_P, _I64, _P, _P]),
+    "prt_grain_mask": (_I, [_P, _I64, _I, _I, _I, _P, _P]),
+    "prt_keep_inlet_connected": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _I,
+                                      ctypes.POINTER(ctypes.c_int), _P]),
+    "prt_coat_with_biofilm": (_I, [_P, _I, _I, _I, _I, _I, _I, _P, _P, _P,
+                                   ctypes.POINTER(ctypes.c_int), _P]),
     "prt_pairwise_plan": (_I, [_I64, _P, _I64, ctypes.POINTER(_I64),
                                ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "prt_pairwise_sum_f32": (_I, [_P, _I64, _I, _P, _I64, _P, _I64, _I64, _I64, _P, _P, _P]),

@@ -24,6 +24,7 @@ SOURCES = {
     "prt_adr.hip": ["-ffp-contract=off"],
     "prt_reduce.hip": ["-ffp-contract=off"],
     "prt_batch.hip": [],
+    "prt_geom.hip": ["-ffp-contract=off"],
 }
 
 

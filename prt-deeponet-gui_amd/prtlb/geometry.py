@@ -6,6 +6,8 @@ are this build's own seeded generators: the reference's dataset generators
 threshold Gaussian fields, so overlapping disks/spheres with periodic
 minimum-image distances are defined here, once, for every workload.
 """
+import ctypes
+
 import numpy as np
 
 SOLID, WALL, PORE = 0, 1, 2
@@ -100,14 +102,140 @@ def _grain_pack(shape, radius, porosity, seed, max_grains=10 ** 7):
     return np.where(solid, SOLID, PORE).astype(np.uint8)
 
 
-def disk_pack(shape=(128, 128), radius=6.0, porosity=0.6, seed=0, connect=True):
-    """Random overlapping solid disks (2D), the 128x128 r=6 phi~0.6 workload."""
-    g = _grain_pack(shape, radius, porosity, seed)
-    return keep_inlet_connected(g)[0] if connect else g
+def _candidates(shape, radius, seed, k):
+    """The first k grains the seeded generator draws, in order."""
+    rng = np.random.default_rng(seed)
+    nd = len(shape)
+    cs = np.empty((k, nd))
+    rs = np.empty(k)
+    ext = np.array(shape, np.float64)
+    for i in range(k):
+        cs[i] = rng.uniform(0.0, 1.0, size=nd) * ext
+        rs[i] = float(radius) if np.isscalar(radius) else float(rng.uniform(*radius))
+    return cs, rs
+
+
+def _cell_lists(shape, cs, rs, cell):
+    """Per cell of edge `cell`, the grains whose periodic bounding box meets
+    it, in ascending grain order (CSR: start offsets, items)."""
+    nd = len(shape)
+    ncell = [(n + cell - 1) // cell for n in shape]
+    pairs_cell, pairs_k = [], []
+    for k in range(len(rs)):
+        axes = []
+        for a, n in enumerate(shape):
+            v = np.arange(int(np.floor(cs[k, a] - rs[k])), int(np.ceil(cs[k, a] + rs[k])) + 1)
+            axes.append(np.unique(np.mod(v, n) // cell))
+        grid = np.ix_(*axes)
+        flat = np.zeros([len(x) for x in axes], np.int64)
+        mul = 1
+        for a in range(nd - 1, -1, -1):
+            flat = flat + grid[a] * mul
+            mul *= ncell[a]
+        flat = flat.ravel()
+        pairs_cell.append(flat)
+        pairs_k.append(np.full(flat.size, k, np.int32))
+    cells = np.concatenate(pairs_cell)
+    ks = np.concatenate(pairs_k)
+    order = np.lexsort((ks, cells))
+    cells, ks = cells[order], ks[order]
+    start = np.zeros(int(np.prod(ncell)) + 1, np.int32)
+    np.add.at(start, cells + 1, 1)
+    return np.cumsum(start).astype(np.int32), ks
+
+
+def grain_pack_device(shape, radius, porosity, seed, device=None, cell=16):
+    """The seeded grain pack of `_grain_pack`, built on the GPU; returns the
+    material grid as a uint8 device tensor, bit-identical to the host one."""
+    import torch
+    from ._lib import call
+    from .device import device as _dev
+    from .device import ptr, stream_ptr
+    dev = _dev(device)
+    shape = tuple(int(s) for s in shape)
+    nd = len(shape)
+    N = int(np.prod(shape))
+    dims = (shape[0], shape[1], 1 if nd == 2 else shape[2])
+    mean_r = float(radius) if np.isscalar(radius) else 0.5 * (radius[0] + radius[1])
+    vol = np.pi * mean_r ** 2 if nd == 2 else 4.0 / 3.0 * np.pi * mean_r ** 3
+    k = int(1.5 * max(-np.log(max(porosity, 1e-6)), 0.05) * N / vol) + 64
+    s = stream_ptr(dev)
+    while True:
+        cs, rs = _candidates(shape, radius, seed, k)
+        start, items = _cell_lists(shape, cs, rs, cell)
+        t_cs = torch.from_numpy(cs).to(dev)
+        t_rs = torch.from_numpy(rs).to(dev)
+        t_start = torch.from_numpy(start).to(dev)
+        t_items = torch.from_numpy(items).to(dev)
+        first = torch.empty(N, dtype=torch.int32, device=dev)
+        call("prt_grain_first_cover", *dims, ptr(t_cs), ptr(t_rs), k, ptr(t_start), ptr(t_items),
+             cell, ptr(first), s)
+        hist = torch.zeros(k + 1, dtype=torch.int32, device=dev)
+        call("prt_grain_histogram", ptr(first), N, ptr(hist), s)
+        h = hist.cpu().numpy().astype(np.int64)
+        solid_after = np.concatenate([[0], np.cumsum(h[:k])])     # solid voxels after j grains
+        ok = np.flatnonzero((N - solid_after) <= porosity * N)
+        if ok.size:
+            kstar = int(ok[0])
+            break
+        k *= 2
+    g = torch.empty(N, dtype=torch.uint8, device=dev)
+    call("prt_grain_mask", ptr(first), N, kstar, SOLID, PORE, ptr(g), s)
+    return g.reshape(shape)
+
+
+def keep_inlet_connected_device(g, max_sweeps=1 << 20):
+    """keep_inlet_connected on a uint8 device tensor, in place; returns
+    (g, n_removed)."""
+    import torch
+    from ._lib import call
+    from .device import ptr, stream_ptr
+    shape = tuple(g.shape)
+    dims = (shape[0], shape[1], 1 if len(shape) == 2 else shape[2])
+    reach = torch.empty(g.numel(), dtype=torch.uint8, device=g.device)
+    scratch = torch.zeros(2, dtype=torch.int32, device=g.device)
+    removed = ctypes.c_int()
+    call("prt_keep_inlet_connected", ptr(g), *dims, PORE, SOLID, ptr(reach), ptr(scratch),
+         int(max_sweeps), ctypes.byref(removed), stream_ptr(g.device))
+    return g, int(removed.value)
+
+
+def coat_with_biofilm_device(g, code, layers=1):
+    """coat_with_biofilm on a uint8 device tensor, in place; (g, n_marked)."""
+    import torch
+    from ._lib import call
+    from .device import ptr, stream_ptr
+    if code is None or layers <= 0:
+        return g, 0
+    shape = tuple(g.shape)
+    dims = (shape[0], shape[1], 1 if len(shape) == 2 else shape[2])
+    wa = torch.empty(g.numel(), dtype=torch.uint8, device=g.device)
+    wb = torch.empty_like(wa)
+    scratch = torch.zeros(1, dtype=torch.int32, device=g.device)
+    marked = ctypes.c_int()
+    call("prt_coat_with_biofilm", ptr(g), *dims, PORE, int(code), int(layers), ptr(wa), ptr(wb),
+         ptr(scratch), ctypes.byref(marked), stream_ptr(g.device))
+    return g, int(marked.value)
+
+
+def _pack(shape, radius, porosity, seed, connect, device):
+    if device is None:
+        g = _grain_pack(shape, radius, porosity, seed)
+        return keep_inlet_connected(g)[0] if connect else g
+    g = grain_pack_device(shape, radius, porosity, seed, device=device)
+    if connect:
+        keep_inlet_connected_device(g)
+    return g.cpu().numpy()
+
+
+def disk_pack(shape=(128, 128), radius=6.0, porosity=0.6, seed=0, connect=True, device=None):
+    """Random overlapping solid disks (2D), the 128x128 r=6 phi~0.6 workload.
+    device: build it on that GPU (bit-identical to the host build)."""
+    return _pack(shape, radius, porosity, seed, connect, device)
 
 
 def sphere_pack(shape=(128, 128, 128), radius=(4.0, 8.0), porosity=0.35, seed=0,
-                connect=True):
-    """Random overlapping solid spheres (3D), radii U(lo, hi)."""
-    g = _grain_pack(shape, radius, porosity, seed)
-    return keep_inlet_connected(g)[0] if connect else g
+                connect=True, device=None):
+    """Random overlapping solid spheres (3D), radii U(lo, hi).
+    device: build it on that GPU (bit-identical to the host build)."""
+    return _pack(shape, radius, porosity, seed, connect, device)
