@@ -49,6 +49,8 @@ extern "C" {
  * per thread along y; 0 = the library's default shape */
 #define PRT_LBM_TUNE(bz_log2, by_log2, npt) \
   (((bz_log2) << 8) | ((by_log2) << 12) | (((npt) - 1) << 16))
+/* nontemporal population loads (dense mode, one node per thread) */
+#define PRT_LBM_NT_LOADS (1 << 18)
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *prt_last_error(void);

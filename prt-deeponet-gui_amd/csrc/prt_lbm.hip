@@ -27,7 +27,7 @@
 
 namespace prt {
 
-template <int Q, int ST, bool SKIP, bool RED, int NPT>
+template <int Q, int ST, bool SKIP, bool RED, int NPT, bool NTL = false>
 __global__ __launch_bounds__(512) void lbm_step_kernel(LbmArgs a) {
   __shared__ double lds[16];
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
@@ -45,12 +45,12 @@ __global__ __launch_bounds__(512) void lbm_step_kernel(LbmArgs a) {
         double ux;
         if constexpr (ST == PRT_STORE_FP64) {
           double f[Q], fp[Q];
-          gather<Q, ST, SKIP>(a, nb, n, f);
+          gather<Q, ST, SKIP, double, NTL>(a, nb, n, f);
           collide_ref<Q>(a, solid, f, fp, ux);
           scatter<Q, ST>(a, n, fp);
         } else {
           float f[Q], fp[Q];
-          gather<Q, ST, SKIP>(a, nb, n, f);
+          gather<Q, ST, SKIP, float, NTL>(a, nb, n, f);
           collide_dev<Q>(a, solid, f, fp, ux);
           scatter<Q, ST>(a, n, fp);
         }
@@ -182,6 +182,7 @@ struct Geom {
   long long N;
   dim3 block, grid;
   int npt;
+  bool ntl;
 };
 
 static int make_geom(int q, int nx, int ny, int nz, Geom &g, int flags = 0) {
@@ -213,6 +214,7 @@ static int make_geom(int q, int nx, int ny, int nz, Geom &g, int flags = 0) {
   while (bz < g.Z && bz < 256) bz <<= 1;
   int by = 256 / bz;
   g.npt = 1;
+  g.ntl = (flags >> 18) & 1;
   const int tz = (flags >> 8) & 15, ty = (flags >> 12) & 15, tn = (flags >> 16) & 3;
   if (tz) {
     bz = 1 << tz;
@@ -273,8 +275,15 @@ static void launch_step_n(const Geom &g, const LbmArgs &a, bool skip, bool red, 
 
 template <int Q, int ST>
 static void launch_step_t(const Geom &g, const LbmArgs &a, bool skip, bool red, hipStream_t st) {
-  if (g.npt == 2) launch_step_n<Q, ST, 2>(g, a, skip, red, st);
-  else launch_step_n<Q, ST, 1>(g, a, skip, red, st);
+  if (g.ntl && !skip && g.npt == 1) {
+    // nontemporal population loads (dense mode, one node per thread)
+    if (red) lbm_step_kernel<Q, ST, false, true, 1, true><<<g.grid, g.block, 0, st>>>(a);
+    else lbm_step_kernel<Q, ST, false, false, 1, true><<<g.grid, g.block, 0, st>>>(a);
+  } else if (g.npt == 2) {
+    launch_step_n<Q, ST, 2>(g, a, skip, red, st);
+  } else {
+    launch_step_n<Q, ST, 1>(g, a, skip, red, st);
+  }
 }
 
 static int launch_step(int q, int storage, const Geom &g, const LbmArgs &a, bool skip, bool red,

@@ -124,7 +124,7 @@ __device__ inline Nbr make_nbr(int x, int y, int z, int X, int Y, int Z) {
 
 // Load the Q populations streaming into node n (SKIP: from the node's own
 // two-step-old opposite slot where the upstream node is solid).
-template <int Q, int ST, bool SKIP, typename R>
+template <int Q, int ST, bool SKIP, typename R, bool NTL = false>
 __device__ inline void gather(const LbmArgs &a, const Nbr &nb, int n, R *f) {
   using T = typename Pop<ST>::T;
   const T *src = static_cast<const T *>(a.src);
@@ -135,6 +135,8 @@ __device__ inline void gather(const LbmArgs &a, const Nbr &nb, int n, R *f) {
     T v;
     if (SKIP && i != 0 && a.solid[s])
       v = own[(size_t)Lat<Q>::opp[i] * a.ld + n];
+    else if constexpr (NTL && ST != PRT_STORE_FP16)
+      v = __builtin_nontemporal_load(src + (size_t)i * a.ld + s);   // read once: do not keep
     else
       v = src[(size_t)i * a.ld + s];
     if constexpr (ST == PRT_STORE_FP16)

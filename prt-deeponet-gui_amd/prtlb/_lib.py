@@ -17,9 +17,12 @@ PRT_STORE = {"fp64": 0, "fp32": 1, "fp16": 2}
 PRT_LBM_SKIP_SOLID = 1
 
 
-def lbm_tune(bz_log2, by_log2, npt):
-    """PRT_LBM_TUNE: launch shape bits for prt_lbm_step / prt_lbm_run."""
-    return (bz_log2 << 8) | (by_log2 << 12) | ((npt - 1) << 16)
+PRT_LBM_NT_LOADS = 1 << 18
+
+
+def lbm_tune(bz_log2, by_log2, npt, nt_loads=False):
+    """PRT_LBM_TUNE (| PRT_LBM_NT_LOADS): launch bits for prt_lbm_step / _run."""
+    return (bz_log2 << 8) | (by_log2 << 12) | ((npt - 1) << 16) | (PRT_LBM_NT_LOADS if nt_loads else 0)
 
 
 class PrtError(RuntimeError):

@@ -15,9 +15,11 @@ from ._lib import lbm_tune
 
 _CACHE = {}
 
-# (log2 threads along z, log2 threads along y, nodes per thread)
-CANDIDATES = [(6, 0, 1), (6, 1, 1), (6, 2, 1), (6, 3, 1), (7, 0, 1), (7, 1, 1), (7, 2, 1),
-              (8, 0, 1), (8, 1, 1), (9, 0, 1), (6, 2, 2), (7, 1, 2), (8, 0, 2), (6, 3, 2)]
+# (log2 threads along z, log2 threads along y, nodes per thread, nontemporal loads)
+CANDIDATES = [(6, 0, 1, 0), (6, 1, 1, 0), (6, 2, 1, 0), (6, 3, 1, 0), (7, 0, 1, 0), (7, 1, 1, 0),
+              (7, 2, 1, 0), (8, 0, 1, 0), (8, 1, 1, 0), (9, 0, 1, 0), (6, 2, 2, 0), (7, 1, 2, 0),
+              (8, 0, 2, 0), (6, 3, 2, 0), (6, 2, 1, 1), (7, 1, 1, 1), (7, 0, 1, 1), (8, 0, 1, 1),
+              (8, 1, 1, 1)]
 
 
 def _key(s):
@@ -71,12 +73,12 @@ def tune(s, steps=10, candidates=None, force=False):
         return _CACHE[key]
     table = {"default": time_shape(s, 0, steps)}
     best, best_ms = 0, table["default"]
-    for bz, by, npt in (candidates or CANDIDATES):
-        if (1 << bz) * (1 << by) > 512:
+    for bz, by, npt, ntl in (candidates or CANDIDATES):
+        if (1 << bz) * (1 << by) > 512 or (ntl and (s.flags or npt != 1)):
             continue
-        t = lbm_tune(bz, by, npt)
+        t = lbm_tune(bz, by, npt, bool(ntl))
         ms = time_shape(s, t, steps)
-        table["%dx%dx%d" % (1 << bz, 1 << by, npt)] = ms
+        table["%dx%dx%d%s" % (1 << bz, 1 << by, npt, "nt" if ntl else "")] = ms
         if ms < best_ms:
             best, best_ms = t, ms
     s.tune = best

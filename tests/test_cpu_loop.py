@@ -1,0 +1,96 @@
+"""The product's event-driven transport loop (AdrRun) against the oracle's
+restatement of the reference loop, with the oracle's step as the device:
+same stop step, same ladder, same picked snapshots, same info.  Also the
+campaign helpers on the host."""
+import numpy as np
+import pytest
+
+from oracle import prtlb_oracle as O
+from prtlb.adr import AdrRun
+from prtlb import campaign as C
+import prtlb
+
+
+class _NumpyDevice:
+    """Stands in for TransportSolver: the oracle's float32 step on the host."""
+
+    def __init__(self, mat, vel, pe, da, ns, chem, da_abio):
+        self.S = O.adr_setup(mat, vel, pe, da, ns, chem, da_abio)
+        S = self.S
+        self.C = O.adr_initial(S)
+        self.dt, self.ns, self.nd, self.shape, self.R = S.dt, S.ns, S.nd, S.m.shape, S.R
+        self.ch, self.umax, self.dmax, self.biofilm = S.ch, S.umax, S.dmax, S.biofilm
+        self.pe_grid, self.l_ref = S.pe_grid, S.l_ref
+        self.surface_voxels = float(S.surf.sum())
+
+    def step_n(self, k):
+        for _ in range(k):
+            self.C = O.adr_step(self.S, self.C)
+
+    def pore_means(self):
+        return np.array([float(self.C[j][self.S.mf].mean()) for j in range(self.ns)])
+
+    def snapshot(self):
+        class _Snap:
+            def __init__(s, a):
+                s.a = a
+
+            def reshape(s, shp):
+                return s
+
+            def cpu(s):
+                return s
+
+            def numpy(s):
+                return s.a
+        return _Snap(np.where(self.S.mf, self.C, 0.0).astype(self.R).reshape((self.ns,) + self.shape))
+
+    def outlet_concentration(self, vel=None):
+        return {"mean": np.zeros(self.ns)}
+
+
+@pytest.mark.parametrize("kw", [
+    dict(pe=100.0, da=1.0, n_t=5, n_species=1),                   # auto stop
+    dict(pe=30.0, da=1.0, n_t=4, n_species=4, steps=300),
+    dict(pe=30.0, da=1.0, n_t=11, n_species=1, steps=200),
+    dict(pe=20.0, da=0.5, n_t=3, n_species=2, max_steps=700),     # cap
+    dict(pe=10.0, da=0.5, n_t=2, n_species=2, steps=0)])
+def test_adrrun_equals_reference_loop(kw):
+    g = np.full((40, 6), O.PORE, np.uint8)
+    v = np.zeros((3, 40, 6), np.float32)
+    v[0] = 1.0
+    kw = dict(kw)
+    n_t, ns = kw.pop("n_t"), kw.pop("n_species")
+    steps, max_steps = kw.pop("steps", None), kw.pop("max_steps", 100000)
+    info_o = {}
+    out_o, tn_o = O.solve_adr(g, v, kw["pe"], kw["da"], n_t, n_species=ns, steps=steps,
+                              max_steps=max_steps, info=info_o)
+    T = _NumpyDevice(g, v, kw["pe"], kw["da"], ns, None, 0.0)
+    run = AdrRun(T, n_t, steps=steps, max_steps=max_steps)
+    while not run.finished:
+        tgt = run.next_target()
+        T.step_n(tgt - run.n)
+        run.arrive(tgt)
+    info = {}
+    out, tn = run.result(info, kw["da"], 0.0)
+    assert np.array_equal(out, out_o) and np.array_equal(tn, tn_o)
+    for k in ("converged", "steps", "ladder", "repeated_snapshots", "dt"):
+        assert info[k] == info_o[k], k
+
+
+def test_campaign_helpers():
+    a = C.campaign_specs(n_geom=5, seed=3)
+    b = C.campaign_specs(n_geom=5, seed=3)
+    assert a == b and all(48 <= s["shape"][0] <= 160 for s in a)
+    assert all(0.1 <= s["pe"] <= 100 and 0.01 <= s["da"] <= 10 for s in a)
+    g = np.full((10, 6, 6), prtlb.PORE, np.uint8)
+    assert C.percolates(g)
+    g[5] = prtlb.SOLID
+    assert not C.percolates(g)
+    v = np.zeros((3, 4, 4), np.float32)
+    v[0] = 0.05
+    f = C.lowered_force(v, 2e-6)
+    assert f == pytest.approx(1e-6 * (0.02 / np.sqrt(3.0)) / 0.05)
+    v[0] = 1e-6
+    assert C.lowered_force(v, 2e-6) == 2e-6          # never raised
+    assert C.pressure_for_peclet(1e-5, 0.01, 20.0, 64, 0.5) == pytest.approx(1e-5 * 20 / (0.01 * 64 / 0.5))
