@@ -61,12 +61,17 @@ def time_lbm(g, storage, steps, warmup, skip_solid=False, tune=True):
     return out
 
 
-def time_adr(g, steps, warmup):
+def time_adr(g, steps, warmup, kinetics="monod"):
     import numpy as np
     import torch
     import prtlb
     v = prtlb.stokes_d3q19(g, nit=1500, force=2e-6, tol=1e-5, storage="fp32")
-    T = prtlb.TransportSolver(g, v, 10.0, 1.0, 1, chem=prtlb.chemistry.monod(0.5))
+    if kinetics == "monod":
+        T = prtlb.TransportSolver(g, v, 10.0, 1.0, 1, chem=prtlb.chemistry.monod(0.5))
+    else:
+        T = prtlb.TransportSolver(g, v, 10.0, 0.0, 2, chem=prtlb.chemistry.sorption_irreversible(),
+                                  da_abio=1.0)
+    T.use_graph = False
     T.step_n(warmup)
     torch.cuda.synchronize()
     st = torch.cuda.current_stream()
@@ -77,7 +82,8 @@ def time_adr(g, steps, warmup):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
     vps = T.n_pore / (ms * 1e-3)
-    bpv = 2 * 1 * 4 + 3 * 4 + 1          # 21 B per pore voxel-step (ns = 1, 3D, f32)
+    mobile = sum(int(T.params.mobile[j]) for j in range(T.ns))
+    bpv = 2 * T.ns * 4 + 3 * 4 + 1       # 21 B per pore voxel-step for ns = 1 (3D, f32)
     return dict(ms_per_step=ms, pore_voxel_steps_per_s=vps, gbs=vps * bpv / 1e9,
                 frac=vps * bpv / PEAK_HBM, bytes_per_voxel=bpv, pore_voxels=T.n_pore)
 
@@ -253,6 +259,8 @@ def main():
     ap.add_argument("--serve", type=int, default=0,
                     help="serving study with this many Poisson requests")
     ap.add_argument("--adr-steps", type=int, default=None)
+    ap.add_argument("--adr-only", action="store_true",
+                    help="only the transport step (128^3 and 256^3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -264,6 +272,14 @@ def main():
     assert torch.cuda.is_available(), "bench needs a HIP device"
     if args.stream_only:
         print(json.dumps({"stream_copy": stream_copy()}), flush=True)
+        return
+    if args.adr_only:
+        res = {}
+        for E in (128, 256):
+            g = prtlb.sphere_pack((E, E, E), (4.0, 8.0), 0.35, seed=0)
+            for kin in ("monod", "sorption"):
+                res["adr_%d_%s" % (E, kin)] = time_adr(g, args.steps, args.warmup, kin)
+        print(json.dumps({"adr": res}), flush=True)
         return
     if args.precision:
         print(json.dumps({"precision": precision_study()}), flush=True)
