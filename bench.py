@@ -61,7 +61,7 @@ def time_lbm(g, storage, steps, warmup, skip_solid=False, tune=True):
     return out
 
 
-def time_adr(g, steps, warmup, kinetics="monod"):
+def time_adr(g, steps, warmup, kinetics="monod", pore_list=True):
     import numpy as np
     import torch
     import prtlb
@@ -72,6 +72,7 @@ def time_adr(g, steps, warmup, kinetics="monod"):
         T = prtlb.TransportSolver(g, v, 10.0, 0.0, 2, chem=prtlb.chemistry.sorption_irreversible(),
                                   da_abio=1.0)
     T.use_graph = False
+    T.pore_list = pore_list
     T.step_n(warmup)
     torch.cuda.synchronize()
     st = torch.cuda.current_stream()
@@ -278,7 +279,9 @@ def main():
         for E in (128, 256):
             g = prtlb.sphere_pack((E, E, E), (4.0, 8.0), 0.35, seed=0)
             for kin in ("monod", "sorption"):
-                res["adr_%d_%s" % (E, kin)] = time_adr(g, args.steps, args.warmup, kin)
+                for pl in (True, False):
+                    res["adr_%d_%s_%s" % (E, kin, "list" if pl else "dense")] = time_adr(
+                        g, args.steps, args.warmup, kin, pl)
         print(json.dumps({"adr": res}), flush=True)
         return
     if args.precision:

@@ -173,6 +173,23 @@ int prt_adr_step(const prt_adr_params *p, int nx, int ny, int nz,
                  const uint8_t *flags, const float *u, const float *c_in,
                  float *c_out, void *stream);
 
+/* `steps` float32 steps with one thread per pore voxel: pore_idx lists the
+ * flat indices of the voxels whose flag bit0 is set, n_pore of them (any
+ * order; C order keeps the loads coalesced).  Same arithmetic, same bits as
+ * prt_adr_run on the listed voxels; no lane works on solid.  Entries that
+ * are not pore voxels, or not below nx*ny*nz, are skipped.  Voxels outside
+ * the list are never written, so write_solid must be 0 (PRT_E_ARG). */
+int prt_adr_run_list(const prt_adr_params *p, int nx, int ny, int nz,
+                     const uint8_t *flags, const float *u,
+                     const int32_t *pore_idx, int64_t n_pore, float *c_a,
+                     float *c_b, int steps, void *stream);
+
+/* One FP64 step over the pore list. */
+int prt_adr_step_list_f64(const prt_adr_params *p, int nx, int ny, int nz,
+                          const uint8_t *flags, const double *u,
+                          const int32_t *pore_idx, int64_t n_pore,
+                          const double *c_in, double *c_out, void *stream);
+
 /* Same step in FP64 (the build's own double-precision variant). */
 int prt_adr_step_f64(const prt_adr_params *p, int nx, int ny, int nz,
                      const uint8_t *flags, const double *u,
@@ -206,6 +223,9 @@ typedef struct prt_adr_segment {
   const uint8_t *flags;           /* device, as prt_adr_step */
   const float *u;
   float *c_a, *c_b;
+  const int32_t *pore_idx;        /* device pore list (as prt_adr_run_list), or
+                                     NULL to cover every voxel */
+  int64_t n_pore;
 } prt_adr_segment;
 
 /* Bytes of one device descriptor of each kind. */

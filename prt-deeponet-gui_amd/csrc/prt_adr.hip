@@ -32,6 +32,24 @@ __global__ __launch_bounds__(256) void adr_step_kernel(AdrArgs<R> a) {
   adr_voxel<R, ND, NSM>(a, x, y, z, cnew);
 }
 
+// One thread per PORE voxel (pidx: the pore voxels' flat indices): no lane
+// of a wave is spent on solid.  Same per-voxel code, so the same bits.
+template <typename R, int ND, int NSM>
+__global__ __launch_bounds__(256) void adr_list_kernel(AdrArgs<R> a, const int32_t *__restrict__ pidx,
+                                                      int npore) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npore) return;
+  const int n = pidx[k];
+  if ((unsigned long long)n >= (unsigned long long)a.N) return;
+  const int yz = a.Y * a.Z;
+  const int x = n / yz;
+  const int r = n - x * yz;
+  const int y = r / a.Z;
+  const int z = r - y * a.Z;
+  R cnew[NSM];
+  adr_voxel<R, ND, NSM>(a, x, y, z, cnew);
+}
+
 // ------------------------------------------------------------------ host side
 template <typename R>
 static int build_args(const prt_adr_params *p, int nx, int ny, int nz, const uint8_t *flags, const R *u,
@@ -59,6 +77,24 @@ static int launch(const prt_adr_params *p, const AdrArgs<R> &a, dim3 g, dim3 b, 
   if (p->nd == 2) launch_nd<R, 2>(a, g, b, st);
   else launch_nd<R, 3>(a, g, b, st);
   return launch_status("prt_adr_step launch");
+}
+
+template <typename R, int ND>
+static void launch_list_nd(const AdrArgs<R> &a, const int32_t *pidx, int npore, hipStream_t st) {
+  const dim3 g((npore + 255) / 256), b(256);
+  if (a.ns == 1) adr_list_kernel<R, ND, 1><<<g, b, 0, st>>>(a, pidx, npore);
+  else if (a.ns == 2) adr_list_kernel<R, ND, 2><<<g, b, 0, st>>>(a, pidx, npore);
+  else if (a.ns <= 4) adr_list_kernel<R, ND, 4><<<g, b, 0, st>>>(a, pidx, npore);
+  else adr_list_kernel<R, ND, 8><<<g, b, 0, st>>>(a, pidx, npore);
+}
+
+template <typename R>
+static int launch_list(const prt_adr_params *p, const AdrArgs<R> &a, const int32_t *pidx, int npore,
+                       hipStream_t st) {
+  if (npore <= 0) return PRT_OK;
+  if (p->nd == 2) launch_list_nd<R, 2>(a, pidx, npore, st);
+  else launch_list_nd<R, 3>(a, pidx, npore, st);
+  return launch_status("prt_adr_list launch");
 }
 
 }  // namespace prt
@@ -100,4 +136,46 @@ extern "C" int prt_adr_run(const prt_adr_params *p, int nx, int ny, int nz, cons
     if (rc) return rc;
   }
   return PRT_OK;
+}
+
+extern "C" int prt_adr_run_list(const prt_adr_params *p, int nx, int ny, int nz, const uint8_t *flags,
+                                const float *u, const int32_t *pore_idx, int64_t n_pore, float *c_a, float *c_b,
+                                int steps, void *stream) {
+  AdrArgs<float> a;
+  dim3 g, b;
+  int rc = build_args<float>(p, nx, ny, nz, flags, u, c_a, c_b, a, g, b);
+  if (rc) return rc;
+  if (steps < 0 || n_pore < 0 || n_pore > a.N || (n_pore > 0 && !pore_idx)) {
+    set_error("prt_adr_run_list: bad step count or pore list");
+    return PRT_E_ARG;
+  }
+  if (p->write_solid) {
+    set_error("prt_adr_run_list: write_solid needs the dense launch (prt_adr_run)");
+    return PRT_E_ARG;
+  }
+  for (int s = 0; s < steps; ++s) {
+    a.cin = (s & 1) ? c_b : c_a;
+    a.cout = (s & 1) ? c_a : c_b;
+    rc = launch_list<float>(p, a, pore_idx, (int)n_pore, (hipStream_t)stream);
+    if (rc) return rc;
+  }
+  return PRT_OK;
+}
+
+extern "C" int prt_adr_step_list_f64(const prt_adr_params *p, int nx, int ny, int nz, const uint8_t *flags,
+                                     const double *u, const int32_t *pore_idx, int64_t n_pore,
+                                     const double *c_in, double *c_out, void *stream) {
+  AdrArgs<double> a;
+  dim3 g, b;
+  int rc = build_args<double>(p, nx, ny, nz, flags, u, c_in, c_out, a, g, b);
+  if (rc) return rc;
+  if (n_pore < 0 || n_pore > a.N || (n_pore > 0 && !pore_idx)) {
+    set_error("prt_adr_step_list_f64: bad pore list");
+    return PRT_E_ARG;
+  }
+  if (p->write_solid) {
+    set_error("prt_adr_step_list_f64: write_solid needs the dense launch (prt_adr_step_f64)");
+    return PRT_E_ARG;
+  }
+  return launch_list<double>(p, a, pore_idx, (int)n_pore, (hipStream_t)stream);
 }

@@ -34,6 +34,8 @@ struct SegLbm {
 struct SegAdr {
   AdrArgs<float> a;
   float *ca, *cb;
+  const int32_t *pidx;   // pore voxels (flat indices); NULL = every voxel
+  int npts;              // entries the blocks cover (pore count, or N)
   int phase, blk0, nblk, nd;
 };
 
@@ -97,10 +99,11 @@ __global__ __launch_bounds__(256) void adr_batch_kernel(const SegAdr *__restrict
   const bool newest_a = ((parity ^ d.phase) & 1) == 0;
   a.cin = newest_a ? d.ca : d.cb;
   a.cout = newest_a ? d.cb : d.ca;
-  const int n = (int)(blockIdx.x - d.blk0) * 256 + threadIdx.x;
+  const int k = (int)(blockIdx.x - d.blk0) * 256 + threadIdx.x;
   float cnew[NSM];
   bool pore = false;
-  if (n < a.N) {
+  const int n = k < d.npts ? (d.pidx ? d.pidx[k] : k) : -1;
+  if (n >= 0 && n < a.N) {
     int x, y, z;
     unflatten(n, a.Y, a.Z, x, y, z);
     pore = d.nd == 3 ? adr_voxel<float, 3, NSM>(a, x, y, z, cnew) : adr_voxel<float, 2, NSM>(a, x, y, z, cnew);
@@ -256,10 +259,16 @@ extern "C" int prt_adr_batch_plan(const prt_adr_segment *segs, int nseg, void *d
     d[s].a.u = g.u;
     d[s].ca = g.c_a;
     d[s].cb = g.c_b;
+    d[s].pidx = g.pore_idx;
+    if (g.pore_idx && (g.n_pore < 0 || g.n_pore > d[s].a.N || g.params->write_solid)) {
+      set_error("prt_adr_batch_plan: segment %d pore list: count out of range or write_solid set", s);
+      return PRT_E_ARG;
+    }
+    d[s].npts = g.pore_idx ? (int)g.n_pore : (int)d[s].a.N;
     d[s].phase = g.phase & 1;
     d[s].nd = g.params->nd;
     d[s].blk0 = (int)total;
-    d[s].nblk = blocks_of(d[s].a.N);
+    d[s].nblk = d[s].npts > 0 ? blocks_of(d[s].npts) : 0;
     total += d[s].nblk;
     nsm = g.params->ns > nsm ? g.params->ns : nsm;
   }

@@ -260,6 +260,7 @@ class TransportSolver:
         self._plan = None
         self.means_scratch = None
         self.use_graph = self.n < (1 << 20)
+        self.pore_list = True
 
     @staticmethod
     def _surface_sum(m, R):
@@ -284,15 +285,24 @@ class TransportSolver:
 
     GRAPH_STEPS = 32
 
+    def _launch_run(self, a, b, steps):
+        """`steps` float32 steps a -> b -> ... (pore-list launch by default)."""
+        pp = ctypes.byref(self.params)
+        if self.pore_list:
+            call("prt_adr_run_list", pp, *self.dims, ptr(self.flags), ptr(self.u),
+                 ptr(self.pore_idx), self.n_pore, ptr(a), ptr(b), int(steps), stream_ptr(self.dev))
+        else:
+            call("prt_adr_run", pp, *self.dims, ptr(self.flags), ptr(self.u), ptr(a), ptr(b),
+                 int(steps), stream_ptr(self.dev))
+
     def _graph_steps(self):
         """Replay GRAPH_STEPS (even) steps from a captured HIP graph."""
-        key = (self.latest,)
+        key = (self.latest, self.pore_list)
         if getattr(self, "_graph_key", None) != key:
             a, b = self.c[self.latest], self.c[1 - self.latest]
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                call("prt_adr_run", ctypes.byref(self.params), *self.dims, ptr(self.flags),
-                     ptr(self.u), ptr(a), ptr(b), self.GRAPH_STEPS, stream_ptr(self.dev))
+                self._launch_run(a, b, self.GRAPH_STEPS)
             self._graph, self._graph_key = g, key
         self._graph.replay()
         self.steps_done += self.GRAPH_STEPS
@@ -311,13 +321,12 @@ class TransportSolver:
         a, b = self.c[self.latest], self.c[1 - self.latest]
         pp = ctypes.byref(self.params)
         if self.precision == "fp32":
-            call("prt_adr_run", pp, *self.dims, ptr(self.flags), ptr(self.u), ptr(a), ptr(b),
-                 int(steps), self._stream())
+            self._launch_run(a, b, int(steps))
         else:
             for s in range(steps):
                 src, dst = (a, b) if s % 2 == 0 else (b, a)
-                call("prt_adr_step_f64", pp, *self.dims, ptr(self.flags), ptr(self.u),
-                     ptr(src), ptr(dst), self._stream())
+                call("prt_adr_step_list_f64", pp, *self.dims, ptr(self.flags), ptr(self.u),
+                     ptr(self.pore_idx), self.n_pore, ptr(src), ptr(dst), self._stream())
         if steps % 2:
             self.latest = 1 - self.latest
         self.steps_done += steps
