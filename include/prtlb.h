@@ -173,21 +173,38 @@ int prt_adr_step(const prt_adr_params *p, int nx, int ny, int nz,
                  const uint8_t *flags, const float *u, const float *c_in,
                  float *c_out, void *stream);
 
-/* `steps` float32 steps with one thread per pore voxel: pore_idx lists the
- * flat indices of the voxels whose flag bit0 is set, n_pore of them (any
- * order; C order keeps the loads coalesced).  Same arithmetic, same bits as
- * prt_adr_run on the listed voxels; no lane works on solid.  Entries that
- * are not pore voxels, or not below nx*ny*nz, are skipped.  Voxels outside
- * the list are never written, so write_solid must be 0 (PRT_E_ARG). */
-int prt_adr_run_list(const prt_adr_params *p, int nx, int ny, int nz,
-                     const uint8_t *flags, const float *u,
-                     const int32_t *pore_idx, int64_t n_pore, float *c_a,
-                     float *c_b, int steps, void *stream);
+/* Pore table of a pore list, built once per transport solve (it depends on
+ * the flags and the velocity only).  pore_idx holds n_pore flat voxel indices
+ * (C order keeps every later load coalesced); for entry k the call writes
+ * word[k] (the open/far/biofilm face bits and edge codes of that voxel) and
+ * its face velocities: face[4k .. 4k+3] = (u_lo, u_hi) of axes 0 and 1, and
+ * in 3D face[4 n_pore + 2k .. +1] = (u_lo, u_hi) of axis 2 (face holds
+ * 2 nd n_pore values).  An entry that is not a pore voxel, or not below
+ * nx*ny*nz, gets word 0xFFFFFFFF and is skipped by the list steps.  The face
+ * velocities are the reference's open-face averages (3D/tools/prtlb_3d.py:
+ * 1100-1135), computed with the same operations as prt_adr_step. */
+int prt_adr_pore_table(int nd, int nx, int ny, int nz, const uint8_t *flags,
+                       const float *u, const int32_t *pore_idx, int64_t n_pore,
+                       uint32_t *word, float *face, void *stream);
+int prt_adr_pore_table_f64(int nd, int nx, int ny, int nz,
+                           const uint8_t *flags, const double *u,
+                           const int32_t *pore_idx, int64_t n_pore,
+                           uint32_t *word, double *face, void *stream);
 
-/* One FP64 step over the pore list. */
+/* `steps` float32 steps with one thread per pore-list entry, geometry from
+ * the pore table of the same list and grid: no lane works on solid, and no
+ * flag or velocity grid is read.  Same arithmetic, same bits as prt_adr_run
+ * on the listed voxels; voxels outside the list are never written, so
+ * write_solid must be 0 (PRT_E_ARG). */
+int prt_adr_run_list(const prt_adr_params *p, int nx, int ny, int nz,
+                     const int32_t *pore_idx, const uint32_t *word,
+                     const float *face, int64_t n_pore, float *c_a, float *c_b,
+                     int steps, void *stream);
+
+/* One FP64 step over the pore list (FP64 pore table). */
 int prt_adr_step_list_f64(const prt_adr_params *p, int nx, int ny, int nz,
-                          const uint8_t *flags, const double *u,
-                          const int32_t *pore_idx, int64_t n_pore,
+                          const int32_t *pore_idx, const uint32_t *word,
+                          const double *face, int64_t n_pore,
                           const double *c_in, double *c_out, void *stream);
 
 /* Same step in FP64 (the build's own double-precision variant). */
@@ -220,12 +237,11 @@ typedef struct prt_lbm_segment {
 typedef struct prt_adr_segment {
   const prt_adr_params *params;   /* host pointer, read by the plan call */
   int32_t nx, ny, nz, phase;
-  const uint8_t *flags;           /* device, as prt_adr_step */
-  const float *u;
-  float *c_a, *c_b;
-  const int32_t *pore_idx;        /* device pore list (as prt_adr_run_list), or
-                                     NULL to cover every voxel */
+  const int32_t *pore_idx;        /* device pore list and its pore table */
+  const uint32_t *word;           /* (prt_adr_pore_table) */
+  const float *face;
   int64_t n_pore;
+  float *c_a, *c_b;               /* device concentration buffers */
 } prt_adr_segment;
 
 /* Bytes of one device descriptor of each kind. */

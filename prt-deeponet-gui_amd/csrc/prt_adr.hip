@@ -29,25 +29,48 @@ __global__ __launch_bounds__(256) void adr_step_kernel(AdrArgs<R> a) {
   const int x = blockIdx.z;
   if (z >= a.Z || y >= a.Y) return;
   R cnew[NSM];
-  adr_voxel<R, ND, NSM>(a, x, y, z, cnew);
+  adr_voxel<R, ND, NSM>(a, (uint32_t)((x * a.Y + y) * a.Z + z), x, y, z, cnew);
 }
 
-// One thread per PORE voxel (pidx: the pore voxels' flat indices): no lane
-// of a wave is spent on solid.  Same per-voxel code, so the same bits.
+// One thread per entry of a pore list (no lane of a wave is spent on solid),
+// geometry from the pore table.  Same update code, so the same bits.
 template <typename R, int ND, int NSM>
 __global__ __launch_bounds__(256) void adr_list_kernel(AdrArgs<R> a, const int32_t *__restrict__ pidx,
-                                                      int npore) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+                                                      const uint32_t *__restrict__ word,
+                                                      const R *__restrict__ face, uint32_t npore) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npore) return;
+  R cnew[NSM];
+  adr_entry<R, ND, NSM>(a, pidx, word, face, k, npore, cnew);
+}
+
+// Pore table of a list: the word and face velocities of every entry, or
+// kPoreWordNone (and zero faces) for an entry that is not a pore voxel.
+template <typename R, int ND>
+__global__ __launch_bounds__(256) void adr_table_kernel(AdrArgs<R> a, const int32_t *__restrict__ pidx,
+                                                       uint32_t npore, uint32_t *__restrict__ word,
+                                                       R *__restrict__ face) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= npore) return;
   const int n = pidx[k];
-  if ((unsigned long long)n >= (unsigned long long)a.N) return;
-  const int yz = a.Y * a.Z;
-  const int x = n / yz;
-  const int r = n - x * yz;
-  const int y = r / a.Z;
-  const int z = r - y * a.Z;
-  R cnew[NSM];
-  adr_voxel<R, ND, NSM>(a, x, y, z, cnew);
+  uint32_t w = kPoreWordNone;
+  vec4<R> f01 = {R(0), R(0), R(0), R(0)};
+  vec2<R> f2 = {R(0), R(0)};
+  if (n >= 0 && (long long)n < a.N) {
+    const uint8_t fl = a.flags[n];
+    if (fl & 1) {
+      int x, y, z;
+      adr_coords(a, n, x, y, z);
+      VoxGeom<R, ND> g;
+      geom_dense<R, ND>(a, (uint32_t)n, fl, x, y, z, g);
+      w = g.w;
+      f01 = vec4<R>{g.u_lo[0], g.u_hi[0], g.u_lo[1], g.u_hi[1]};
+      f2 = vec2<R>{g.u_lo[ND - 1], g.u_hi[ND - 1]};
+    }
+  }
+  word[k] = w;
+  reinterpret_cast<vec4<R> *>(face)[k] = f01;
+  if (ND == 3) reinterpret_cast<vec2<R> *>(face + (size_t)4 * npore)[k] = f2;
 }
 
 // ------------------------------------------------------------------ host side
@@ -80,21 +103,62 @@ static int launch(const prt_adr_params *p, const AdrArgs<R> &a, dim3 g, dim3 b, 
 }
 
 template <typename R, int ND>
-static void launch_list_nd(const AdrArgs<R> &a, const int32_t *pidx, int npore, hipStream_t st) {
+static void launch_list_nd(const AdrArgs<R> &a, const int32_t *pidx, const uint32_t *word, const R *face,
+                           uint32_t npore, hipStream_t st) {
   const dim3 g((npore + 255) / 256), b(256);
-  if (a.ns == 1) adr_list_kernel<R, ND, 1><<<g, b, 0, st>>>(a, pidx, npore);
-  else if (a.ns == 2) adr_list_kernel<R, ND, 2><<<g, b, 0, st>>>(a, pidx, npore);
-  else if (a.ns <= 4) adr_list_kernel<R, ND, 4><<<g, b, 0, st>>>(a, pidx, npore);
-  else adr_list_kernel<R, ND, 8><<<g, b, 0, st>>>(a, pidx, npore);
+  if (a.ns == 1) adr_list_kernel<R, ND, 1><<<g, b, 0, st>>>(a, pidx, word, face, npore);
+  else if (a.ns == 2) adr_list_kernel<R, ND, 2><<<g, b, 0, st>>>(a, pidx, word, face, npore);
+  else if (a.ns <= 4) adr_list_kernel<R, ND, 4><<<g, b, 0, st>>>(a, pidx, word, face, npore);
+  else adr_list_kernel<R, ND, 8><<<g, b, 0, st>>>(a, pidx, word, face, npore);
+}
+
+// Arguments of a list launch: parameters, grid, pore list and table.
+template <typename R>
+static int list_args(const char *who, const prt_adr_params *p, int nx, int ny, int nz, const int32_t *pidx,
+                     const uint32_t *word, const R *face, int64_t n_pore, const R *cin, R *cout,
+                     AdrArgs<R> &a) {
+  dim3 g, b;
+  const int rc = adr_args_from_params<R>(p, nx, ny, nz, a, g, b);
+  if (rc) return rc;
+  if (!cin || !cout || cin == cout || n_pore < 0 || n_pore > a.N ||
+      (n_pore > 0 && (!pidx || !word || !face))) {
+    set_error("%s: null or aliased buffers, or a bad pore count", who);
+    return PRT_E_ARG;
+  }
+  if ((unsigned long long)n_pore * 4ull * sizeof(R) >= (1ull << 32)) {
+    set_error("%s: pore list too long for the 32-bit table offsets", who);
+    return PRT_E_RANGE;
+  }
+  if (p->write_solid) {
+    set_error("%s: write_solid needs the dense launch", who);
+    return PRT_E_ARG;
+  }
+  a.cin = cin;
+  a.cout = cout;
+  return PRT_OK;
 }
 
 template <typename R>
-static int launch_list(const prt_adr_params *p, const AdrArgs<R> &a, const int32_t *pidx, int npore,
-                       hipStream_t st) {
-  if (npore <= 0) return PRT_OK;
-  if (p->nd == 2) launch_list_nd<R, 2>(a, pidx, npore, st);
-  else launch_list_nd<R, 3>(a, pidx, npore, st);
-  return launch_status("prt_adr_list launch");
+static int table(int nd, int nx, int ny, int nz, const uint8_t *flags, const R *u, const int32_t *pidx,
+                 int64_t n_pore, uint32_t *word, R *face, hipStream_t st) {
+  AdrArgs<R> a;
+  const int rc = adr_grid<R>(nd, nx, ny, nz, a);
+  if (rc) return rc;
+  if (n_pore < 0 || n_pore > a.N || (n_pore > 0 && (!flags || !u || !pidx || !word || !face))) {
+    set_error("prt_adr_pore_table: null buffers or a bad pore count");
+    return PRT_E_ARG;
+  }
+  if ((unsigned long long)n_pore * 4ull * sizeof(R) >= (1ull << 32)) {
+    set_error("prt_adr_pore_table: pore list too long for the 32-bit table offsets");
+    return PRT_E_RANGE;
+  }
+  if (n_pore == 0) return PRT_OK;
+  a.flags = flags;
+  a.u = u;
+  const dim3 g((unsigned)((n_pore + 255) / 256)), b(256);
+  if (nd == 2) adr_table_kernel<R, 2><<<g, b, 0, st>>>(a, pidx, (uint32_t)n_pore, word, face);
+  else adr_table_kernel<R, 3><<<g, b, 0, st>>>(a, pidx, (uint32_t)n_pore, word, face);
+  return launch_status("prt_adr_pore_table launch");
 }
 
 }  // namespace prt
@@ -138,44 +202,49 @@ extern "C" int prt_adr_run(const prt_adr_params *p, int nx, int ny, int nz, cons
   return PRT_OK;
 }
 
-extern "C" int prt_adr_run_list(const prt_adr_params *p, int nx, int ny, int nz, const uint8_t *flags,
-                                const float *u, const int32_t *pore_idx, int64_t n_pore, float *c_a, float *c_b,
+extern "C" int prt_adr_pore_table(int nd, int nx, int ny, int nz, const uint8_t *flags, const float *u,
+                                  const int32_t *pore_idx, int64_t n_pore, uint32_t *word, float *face,
+                                  void *stream) {
+  return table<float>(nd, nx, ny, nz, flags, u, pore_idx, n_pore, word, face, (hipStream_t)stream);
+}
+
+extern "C" int prt_adr_pore_table_f64(int nd, int nx, int ny, int nz, const uint8_t *flags, const double *u,
+                                      const int32_t *pore_idx, int64_t n_pore, uint32_t *word, double *face,
+                                      void *stream) {
+  return table<double>(nd, nx, ny, nz, flags, u, pore_idx, n_pore, word, face, (hipStream_t)stream);
+}
+
+extern "C" int prt_adr_run_list(const prt_adr_params *p, int nx, int ny, int nz, const int32_t *pore_idx,
+                                const uint32_t *word, const float *face, int64_t n_pore, float *c_a, float *c_b,
                                 int steps, void *stream) {
   AdrArgs<float> a;
-  dim3 g, b;
-  int rc = build_args<float>(p, nx, ny, nz, flags, u, c_a, c_b, a, g, b);
+  int rc = list_args<float>("prt_adr_run_list", p, nx, ny, nz, pore_idx, word, face, n_pore, c_a, c_b, a);
   if (rc) return rc;
-  if (steps < 0 || n_pore < 0 || n_pore > a.N || (n_pore > 0 && !pore_idx)) {
-    set_error("prt_adr_run_list: bad step count or pore list");
+  if (steps < 0) {
+    set_error("prt_adr_run_list: negative step count");
     return PRT_E_ARG;
   }
-  if (p->write_solid) {
-    set_error("prt_adr_run_list: write_solid needs the dense launch (prt_adr_run)");
-    return PRT_E_ARG;
-  }
+  if (n_pore == 0) return PRT_OK;
   for (int s = 0; s < steps; ++s) {
     a.cin = (s & 1) ? c_b : c_a;
     a.cout = (s & 1) ? c_a : c_b;
-    rc = launch_list<float>(p, a, pore_idx, (int)n_pore, (hipStream_t)stream);
+    if (p->nd == 2) launch_list_nd<float, 2>(a, pore_idx, word, face, (uint32_t)n_pore, (hipStream_t)stream);
+    else launch_list_nd<float, 3>(a, pore_idx, word, face, (uint32_t)n_pore, (hipStream_t)stream);
+    rc = launch_status("prt_adr_run_list launch");
     if (rc) return rc;
   }
   return PRT_OK;
 }
 
-extern "C" int prt_adr_step_list_f64(const prt_adr_params *p, int nx, int ny, int nz, const uint8_t *flags,
-                                     const double *u, const int32_t *pore_idx, int64_t n_pore,
+extern "C" int prt_adr_step_list_f64(const prt_adr_params *p, int nx, int ny, int nz, const int32_t *pore_idx,
+                                     const uint32_t *word, const double *face, int64_t n_pore,
                                      const double *c_in, double *c_out, void *stream) {
   AdrArgs<double> a;
-  dim3 g, b;
-  int rc = build_args<double>(p, nx, ny, nz, flags, u, c_in, c_out, a, g, b);
+  const int rc = list_args<double>("prt_adr_step_list_f64", p, nx, ny, nz, pore_idx, word, face, n_pore, c_in,
+                                   c_out, a);
   if (rc) return rc;
-  if (n_pore < 0 || n_pore > a.N || (n_pore > 0 && !pore_idx)) {
-    set_error("prt_adr_step_list_f64: bad pore list");
-    return PRT_E_ARG;
-  }
-  if (p->write_solid) {
-    set_error("prt_adr_step_list_f64: write_solid needs the dense launch (prt_adr_step_f64)");
-    return PRT_E_ARG;
-  }
-  return launch_list<double>(p, a, pore_idx, (int)n_pore, (hipStream_t)stream);
+  if (n_pore == 0) return PRT_OK;
+  if (p->nd == 2) launch_list_nd<double, 2>(a, pore_idx, word, face, (uint32_t)n_pore, (hipStream_t)stream);
+  else launch_list_nd<double, 3>(a, pore_idx, word, face, (uint32_t)n_pore, (hipStream_t)stream);
+  return launch_status("prt_adr_step_list_f64 launch");
 }

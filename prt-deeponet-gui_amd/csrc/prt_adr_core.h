@@ -24,6 +24,7 @@ struct AdrArgs {
   int sorption, sorb_c, sorb_s, write_solid;
   R dt, k, ka, kd, ks_d, ks_a, alpha_dt, neg_alpha, c2, yield_, c3_init, k_sorb, k_desorb;
   int alpha_pos;
+  FastDiv div_yz, div_z;                 // flat index -> (x, y, z)
   int mobile[8], pinned[8];
   R fed[8];
   R dface[8][4];
@@ -53,60 +54,150 @@ __device__ inline R muscl(R up, R upup, R down) {
   return up + R(0.5) * slope;
 }
 
-// One voxel of the step.  Returns false (and writes nothing unless
-// write_solid) outside the pore; otherwise writes the new values to cout and
-// also returns them in cnew.
-template <typename R, int ND, int NSM>
-__device__ inline bool adr_voxel(const AdrArgs<R> &a, int x, int y, int z, R *cnew) {
-#pragma clang fp contract(off)
-  const int n = (x * a.Y + y) * a.Z + z;
-  const uint8_t fl = a.flags[n];
-  const int ns = a.ns;
-  if (!(fl & 1)) {
-    if (a.write_solid)
-#pragma unroll
-      for (int j = 0; j < NSM; ++j)
-        if (j < ns) a.cout[(size_t)j * a.N + n] = R(0);
-    return false;
-  }
-  const int bf_self = (fl >> 1) & 1;
+// base[i] through a 32-bit byte offset from a wave-uniform base: one scalar
+// base register plus one vector offset, no 64-bit address arithmetic.  The
+// argument check keeps every offset below 2^32.
+template <typename T>
+__device__ inline T ld32(const T *base, uint32_t i) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ inline void st32(T *base, uint32_t i, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + i * (uint32_t)sizeof(T)) = v;
+}
 
-  // Every neighbour value the stencil may need is loaded up front and
-  // unconditionally (indices are clamped, so always in bounds); the face
-  // rules then pick among loaded values.  Loads that depended on neighbour
-  // flags would chain several memory round trips per axis.
-  int nidx[ND][4], nx0[ND][4];           // [axis][m2, m1, p1, p2]
+// Pore-table word of one pore voxel (one uint32):
+//   bits 6a .. 6a+5, axis a = 0 .. ND-1: low face open, high face open, far
+//     low face open, far high face open, low neighbour biofilm, high
+//     neighbour biofilm (a face is open when the neighbour across it is pore);
+//   bit 18: the voxel itself is biofilm;
+//   bits 19+4a, 21+4a: edge codes of axis a, low and high side (0 interior,
+//     1 one voxel from the edge, 2 on the edge), which fix the clamped
+//     neighbour offsets;
+//   bit 31: x == 2 (with the x low code, where the inlet pin reaches).
+// 0xFFFFFFFF (edge code 3) marks a list entry that is not a pore voxel.
+constexpr uint32_t kPoreWordNone = 0xFFFFFFFFu;
+
+// Geometry of one pore voxel for one step: the clamped neighbour indices,
+// the word above and the face velocities.  Built from the flag and velocity
+// grids (dense launch, table build) or read from the pore table (list
+// launch); both give the same values, so the update is bit-identical.
+template <typename R, int ND>
+struct VoxGeom {
+  uint32_t nidx[ND][4];  // [axis][m2, m1, p1, p2]
+  uint32_t w;
+  R u_lo[ND], u_hi[ND];
+};
+
+template <typename R>
+__device__ inline int axis_stride(const AdrArgs<R> &a, int ax3) {
+  return ax3 == 0 ? a.Y * a.Z : (ax3 == 1 ? a.Z : 1);
+}
+
+// Clamped neighbour indices from the edge codes of word w.
+template <typename R, int ND>
+__device__ inline void geom_index(const AdrArgs<R> &a, uint32_t n, VoxGeom<R, ND> &g) {
 #pragma unroll
   for (int ax = 0; ax < ND; ++ax) {
-    // axis geometry: 3D axes x, y, z; a 2D grid is (X, 1, Z) with axes x, z
+    // 3D axes x, y, z; a 2D grid is (X, 1, Z) with axes x, z
+    const int stride = axis_stride(a, (ND == 2 && ax == 1) ? 2 : ax);
+    const uint32_t lo = (g.w >> (19 + 4 * ax)) & 3u, hi = (g.w >> (21 + 4 * ax)) & 3u;
+    const int om1 = lo == 2 ? 0 : stride;
+    const int om2 = lo == 0 ? 2 * stride : om1;
+    const int op1 = hi == 2 ? 0 : stride;
+    const int op2 = hi == 0 ? 2 * stride : op1;
+    g.nidx[ax][0] = n - (uint32_t)om2;
+    g.nidx[ax][1] = n - (uint32_t)om1;
+    g.nidx[ax][2] = n + (uint32_t)op1;
+    g.nidx[ax][3] = n + (uint32_t)op2;
+  }
+}
+
+// Geometry from the grids: pore voxel n = (x*Y + y)*Z + z with flag byte fl.
+template <typename R, int ND>
+__device__ inline void geom_dense(const AdrArgs<R> &a, uint32_t n, uint8_t fl, int x, int y, int z,
+                                  VoxGeom<R, ND> &g) {
+#pragma clang fp contract(off)
+  uint32_t w = ((uint32_t)(fl >> 1) & 1u) << 18;
+  if (x == 2) w |= 1u << 31;
+#pragma unroll
+  for (int ax = 0; ax < ND; ++ax) {
     const int ax3 = (ND == 2 && ax == 1) ? 2 : ax;
-    const int stride = ax3 == 0 ? a.Y * a.Z : (ax3 == 1 ? a.Z : 1);
     const int ext = ax3 == 0 ? a.X : (ax3 == 1 ? a.Y : a.Z);
     const int p = ax3 == 0 ? x : (ax3 == 1 ? y : z);
-    const int pm1 = p > 0 ? p - 1 : 0, pm2 = p > 1 ? p - 2 : 0;
-    const int pp1 = p < ext - 1 ? p + 1 : ext - 1, pp2 = p < ext - 2 ? p + 2 : ext - 1;
-    nidx[ax][0] = n + (pm2 - p) * stride;
-    nidx[ax][1] = n + (pm1 - p) * stride;
-    nidx[ax][2] = n + (pp1 - p) * stride;
-    nidx[ax][3] = n + (pp2 - p) * stride;
-    nx0[ax][0] = ax3 == 0 ? pm2 : x;
-    nx0[ax][1] = ax3 == 0 ? pm1 : x;
-    nx0[ax][2] = ax3 == 0 ? pp1 : x;
-    nx0[ax][3] = ax3 == 0 ? pp2 : x;
+    const uint32_t lo = p == 0 ? 2u : (p == 1 ? 1u : 0u);
+    const uint32_t hi = p == ext - 1 ? 2u : (p == ext - 2 ? 1u : 0u);
+    w |= (lo << (19 + 4 * ax)) | (hi << (21 + 4 * ax));
   }
-  uint8_t nf[ND][4];
-  R um[ND], uc[ND], up[ND];
+  g.w = w;
+  geom_index<R, ND>(a, n, g);
 #pragma unroll
   for (int ax = 0; ax < ND; ++ax) {
+    uint8_t nf[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) nf[ax][k] = a.flags[nidx[ax][k]];
+    for (int k = 0; k < 4; ++k) nf[k] = ld32(a.flags, g.nidx[ax][k]);
     const R *ua = a.u + (size_t)ax * a.N;
-    um[ax] = ua[nidx[ax][1]];
-    uc[ax] = ua[n];
-    up[ax] = ua[nidx[ax][2]];
+    const R um = ld32(ua, g.nidx[ax][1]), uc = ld32(ua, n), up = ld32(ua, g.nidx[ax][2]);
+    const bool o_lo = nf[1] & 1, o_hi = nf[2] & 1;
+    const bool f_lo = o_lo && (nf[0] & 1);
+    const bool f_hi = o_hi && (nf[3] & 1);
+    g.u_lo[ax] = o_lo ? R(0.5) * (uc + um) : R(0);
+    g.u_hi[ax] = o_hi ? R(0.5) * (uc + up) : R(0);
+    g.w |= ((uint32_t)o_lo | (uint32_t)o_hi << 1 | (uint32_t)f_lo << 2 | (uint32_t)f_hi << 3 |
+            (uint32_t)((nf[1] >> 1) & 1) << 4 | (uint32_t)((nf[2] >> 1) & 1) << 5)
+           << (6 * ax);
   }
+}
+
+// Face velocities of list entry k: axes 0 and 1 as one 4-vector
+// (u_lo0, u_hi0, u_lo1, u_hi1) at face[4k], axis 2 (3D) as a 2-vector at
+// face[4*n_pore + 2k].
+template <typename R>
+using vec4 = R __attribute__((ext_vector_type(4)));
+template <typename R>
+using vec2 = R __attribute__((ext_vector_type(2)));
+
+template <typename R, int ND>
+__device__ inline void geom_table(const AdrArgs<R> &a, uint32_t n, uint32_t w, const R *face, uint32_t k,
+                                  uint32_t n_pore, VoxGeom<R, ND> &g) {
+  g.w = w;
+  geom_index<R, ND>(a, n, g);
+  const vec4<R> f01 = ld32(reinterpret_cast<const vec4<R> *>(face), k);
+  g.u_lo[0] = f01.x;
+  g.u_hi[0] = f01.y;
+  g.u_lo[1] = f01.z;
+  g.u_hi[1] = f01.w;
+  if (ND == 3) {
+    const vec2<R> f2 = ld32(reinterpret_cast<const vec2<R> *>(face + (size_t)4 * n_pore), k);
+    g.u_lo[ND - 1] = f2.x;
+    g.u_hi[ND - 1] = f2.y;
+  }
+}
+
+// The step at one pore voxel n with geometry g: writes the new values to
+// cout and returns them in cnew.
+template <typename R, int ND, int NSM>
+__device__ inline void adr_update(const AdrArgs<R> &a, uint32_t n, const VoxGeom<R, ND> &g, R *cnew) {
+#pragma clang fp contract(off)
+  const int ns = a.ns;
+  const uint32_t w = g.w;
+  const int bf_self = (w >> 18) & 1;
   // the inlet pin (3D/tools/prtlb_3d.py:1398-1400) applied as the value is
   // read: a voxel on x = 0 that is read at all is a pore voxel
+  const uint32_t xlo = (w >> 19) & 3u;
+  const bool x_is0 = xlo == 2, x_le1 = xlo != 0, x_le2 = x_le1 || (w >> 31);
+  bool onx0[ND][4];
+#pragma unroll
+  for (int ax = 0; ax < ND; ++ax)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) onx0[ax][k] = x_is0;
+  onx0[0][0] = x_le2;
+  onx0[0][1] = x_le1;
+  onx0[0][2] = a.X == 1;
+  onx0[0][3] = a.X == 1;
+
+  // every neighbour value is loaded up front and unconditionally (indices
+  // are clamped, so always in bounds); the face rules pick among them
   R c[NSM], tr[NSM], cn[NSM][ND][4];
 #pragma unroll
   for (int j = 0; j < NSM; ++j) {
@@ -114,29 +205,26 @@ __device__ inline bool adr_voxel(const AdrArgs<R> &a, int x, int y, int z, R *cn
     c[j] = R(0);
     if (j >= ns) continue;
     const R *cj = a.cin + (size_t)j * a.N;
-    const R v = cj[n];
-    c[j] = (x == 0 && a.pinned[j]) ? a.fed[j] : v;
+    const R v = ld32(cj, n);
+    c[j] = (x_is0 && a.pinned[j]) ? a.fed[j] : v;
     if (!a.mobile[j]) continue;
 #pragma unroll
     for (int ax = 0; ax < ND; ++ax)
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const R w = cj[nidx[ax][k]];
-        cn[j][ax][k] = (nx0[ax][k] == 0 && a.pinned[j]) ? a.fed[j] : w;
+        const R v2 = ld32(cj, g.nidx[ax][k]);
+        cn[j][ax][k] = (onx0[ax][k] && a.pinned[j]) ? a.fed[j] : v2;
       }
   }
 
   R divu = R(0);
 #pragma unroll
   for (int ax = 0; ax < ND; ++ax) {
-    const uint8_t fm1 = nf[ax][1], fp1 = nf[ax][2];
-    const bool o_lo = fm1 & 1, o_hi = fp1 & 1;
-    const bool f_lo = o_lo && (nf[ax][0] & 1);
-    const bool f_hi = o_hi && (nf[ax][3] & 1);
-    const R u_lo = o_lo ? R(0.5) * (uc[ax] + um[ax]) : R(0);
-    const R u_hi = o_hi ? R(0.5) * (uc[ax] + up[ax]) : R(0);
+    const uint32_t fb = w >> (6 * ax);
+    const bool o_lo = fb & 1, o_hi = (fb >> 1) & 1, f_lo = (fb >> 2) & 1, f_hi = (fb >> 3) & 1;
+    const bool bm1 = (fb >> 4) & 1, bp1 = (fb >> 5) & 1;
+    const R u_lo = g.u_lo[ax], u_hi = g.u_hi[ax];
     divu = (ax == 0) ? R(0) + (u_hi - u_lo) : divu + (u_hi - u_lo);
-    const int bm1 = (fm1 >> 1) & 1, bp1 = (fp1 >> 1) & 1;
 #pragma unroll
     for (int j = 0; j < NSM; ++j) {
       if (j >= ns || !a.mobile[j]) continue;
@@ -161,6 +249,11 @@ __device__ inline bool adr_voxel(const AdrArgs<R> &a, int x, int y, int z, R *cn
 #pragma unroll
   for (int j = 0; j < NSM; ++j)
     if (j < ns && a.mobile[j]) tr[j] = tr[j] + c[j] * divu;
+
+  // closed faces over 2*nd, edge-replicated neighbours (:1000-1018)
+  int closed = 0;
+#pragma unroll
+  for (int ax = 0; ax < ND; ++ax) closed += !((w >> (6 * ax)) & 1) + !((w >> (6 * ax + 1)) & 1);
 
   // ---- reactions, 3D/tools/prtlb_3d.py:1224-1367
   R r[NSM];
@@ -187,10 +280,6 @@ __device__ inline bool adr_voxel(const AdrArgs<R> &a, int x, int y, int z, R *cn
       if (j == a.abio_index) cj = c[j];
     R surf = R(1);
     if (a.surface_only) {
-      // closed faces over 2*nd, edge-replicated neighbours (:1000-1018)
-      int closed = 0;
-#pragma unroll
-      for (int ax = 0; ax < ND; ++ax) closed += !(nf[ax][1] & 1) + !(nf[ax][2] & 1);
       surf = R(closed) / R(2 * ND);
       Ra = (a.ka * cj) * surf;
     } else {
@@ -223,12 +312,7 @@ __device__ inline bool adr_voxel(const AdrArgs<R> &a, int x, int y, int z, R *cn
       if (j == a.sorb_s) ss = c[j];
     }
     R surf = R(1);
-    if (a.surface_only) {
-      int closed = 0;
-#pragma unroll
-      for (int ax = 0; ax < ND; ++ax) closed += !(nf[ax][1] & 1) + !(nf[ax][2] & 1);
-      surf = R(closed) / R(2 * ND);
-    }
+    if (a.surface_only) surf = R(closed) / R(2 * ND);
     R fw = (a.k_sorb * cs) * surf;
     fw = npmin(fw, (R(0.5) * cs) / a.dt);
     R bk = a.k_desorb * ss;
@@ -247,10 +331,78 @@ __device__ inline bool adr_voxel(const AdrArgs<R> &a, int x, int y, int z, R *cn
     if (j >= ns) continue;
     const R t = a.mobile[j] ? tr[j] : R(0);
     const R v = npmax(c[j] + a.dt * (t + r[j]), R(0));
-    a.cout[(size_t)j * a.N + n] = v;
+    st32(a.cout + (size_t)j * a.N, n, v);
     cnew[j] = v;
   }
+}
+
+// One voxel (flat index n = (x*Y + y)*Z + z) from the grids.  Returns false
+// (and writes nothing unless write_solid) outside the pore.
+template <typename R, int ND, int NSM>
+__device__ inline bool adr_voxel(const AdrArgs<R> &a, uint32_t n, int x, int y, int z, R *cnew) {
+  const uint8_t fl = ld32(a.flags, n);
+  if (!(fl & 1)) {
+    if (a.write_solid)
+#pragma unroll
+      for (int j = 0; j < NSM; ++j)
+        if (j < a.ns) st32(a.cout + (size_t)j * a.N, n, R(0));
+    return false;
+  }
+  VoxGeom<R, ND> g;
+  geom_dense<R, ND>(a, n, fl, x, y, z, g);
+  adr_update<R, ND, NSM>(a, n, g, cnew);
   return true;
+}
+
+// List entry k from the pore table.  Returns false for an entry marked not
+// pore (kPoreWordNone).
+template <typename R, int ND, int NSM>
+__device__ inline bool adr_entry(const AdrArgs<R> &a, const int32_t *pidx, const uint32_t *word, const R *face,
+                                 uint32_t k, uint32_t n_pore, R *cnew) {
+  const uint32_t w = ld32(word, k);
+  if (w == kPoreWordNone) return false;
+  const uint32_t n = (uint32_t)ld32(pidx, k);
+  VoxGeom<R, ND> g;
+  geom_table<R, ND>(a, n, w, face, k, n_pore, g);
+  adr_update<R, ND, NSM>(a, n, g, cnew);
+  return true;
+}
+
+// Flat voxel index -> (x, y, z).
+template <typename R>
+__device__ inline void adr_coords(const AdrArgs<R> &a, int n, int &x, int &y, int &z) {
+  x = fastdiv(n, a.div_yz);
+  const int r = n - x * a.div_yz.d;
+  y = fastdiv(r, a.div_z);
+  z = r - y * a.Z;
+}
+
+// Validate a grid (nd = 2: nx x ny with nz = 1, stored as (nx, 1, ny)) and
+// set the grid fields of a, zeroing the rest.  Host code.
+template <typename R>
+inline int adr_grid(int nd, int nx, int ny, int nz, AdrArgs<R> &a) {
+  if (nd != 2 && nd != 3) {
+    set_error("prt_adr_step: nd must be 2 or 3 (got %d)", nd);
+    return PRT_E_ARG;
+  }
+  if (nx < 1 || ny < 1 || nz < 1 || (nd == 2 && nz != 1)) {
+    set_error("prt_adr_step: bad grid (%d, %d, %d) for nd = %d", nx, ny, nz, nd);
+    return PRT_E_ARG;
+  }
+  std::memset(&a, 0, sizeof(a));
+  if (nd == 2) {
+    a.X = nx; a.Y = 1; a.Z = ny;
+  } else {
+    a.X = nx; a.Y = ny; a.Z = nz;
+  }
+  a.N = (long long)nx * ny * nz;
+  if (a.N >= (1LL << 31) || a.N * (long long)sizeof(R) >= (1LL << 32) || a.X > 65535) {
+    set_error("prt_adr_step: grid too large for the 32-bit voxel index");
+    return PRT_E_RANGE;
+  }
+  a.div_yz = make_fastdiv(a.Y * a.Z);
+  a.div_z = make_fastdiv(a.Z);
+  return PRT_OK;
 }
 
 // Validate a parameter block and grid and convert them to kernel arguments
@@ -262,18 +414,12 @@ inline int adr_args_from_params(const prt_adr_params *p, int nx, int ny, int nz,
     set_error("prt_adr_step: null parameter block");
     return PRT_E_ARG;
   }
-  if (p->nd != 2 && p->nd != 3) {
-    set_error("prt_adr_step: nd must be 2 or 3 (got %d)", p->nd);
-    return PRT_E_ARG;
-  }
+
   if (p->ns < 1 || p->ns > 8) {
     set_error("prt_adr_step: 1..8 chemicals supported (got %d)", p->ns);
     return PRT_E_ARG;
   }
-  if (nx < 1 || ny < 1 || nz < 1 || (p->nd == 2 && nz != 1)) {
-    set_error("prt_adr_step: bad grid (%d, %d, %d) for nd = %d", nx, ny, nz, p->nd);
-    return PRT_E_ARG;
-  }
+
   if (!(p->dt > 0)) {
     set_error("prt_adr_step: dt must be positive");
     return PRT_E_ARG;
@@ -285,14 +431,9 @@ inline int adr_args_from_params(const prt_adr_params *p, int nx, int ny, int nz,
     set_error("prt_adr_step: reaction species index out of range");
     return PRT_E_ARG;
   }
-  std::memset(&a, 0, sizeof(a));
-  if (p->nd == 2) {
-    a.X = nx; a.Y = 1; a.Z = ny;
-  } else {
-    a.X = nx; a.Y = ny; a.Z = nz;
-  }
-  a.N = (long long)nx * ny * nz;
-  if (a.N * p->ns >= (1LL << 31) || a.X > 65535) {
+  const int grc = adr_grid<R>(p->nd, nx, ny, nz, a);
+  if (grc) return grc;
+  if (a.N * p->ns >= (1LL << 31)) {
     set_error("prt_adr_step: grid too large for the 32-bit voxel index");
     return PRT_E_RANGE;
   }

@@ -34,8 +34,10 @@ struct SegLbm {
 struct SegAdr {
   AdrArgs<float> a;
   float *ca, *cb;
-  const int32_t *pidx;   // pore voxels (flat indices); NULL = every voxel
-  int npts;              // entries the blocks cover (pore count, or N)
+  const int32_t *pidx;   // pore list and its pore table
+  const uint32_t *word;
+  const float *face;
+  int npts;              // list entries
   int phase, blk0, nblk, nd;
 };
 
@@ -99,15 +101,12 @@ __global__ __launch_bounds__(256) void adr_batch_kernel(const SegAdr *__restrict
   const bool newest_a = ((parity ^ d.phase) & 1) == 0;
   a.cin = newest_a ? d.ca : d.cb;
   a.cout = newest_a ? d.cb : d.ca;
-  const int k = (int)(blockIdx.x - d.blk0) * 256 + threadIdx.x;
+  const uint32_t k = (uint32_t)(blockIdx.x - d.blk0) * 256 + threadIdx.x;
   float cnew[NSM];
   bool pore = false;
-  const int n = k < d.npts ? (d.pidx ? d.pidx[k] : k) : -1;
-  if (n >= 0 && n < a.N) {
-    int x, y, z;
-    unflatten(n, a.Y, a.Z, x, y, z);
-    pore = d.nd == 3 ? adr_voxel<float, 3, NSM>(a, x, y, z, cnew) : adr_voxel<float, 2, NSM>(a, x, y, z, cnew);
-  }
+  if (k < (uint32_t)d.npts)
+    pore = d.nd == 3 ? adr_entry<float, 3, NSM>(a, d.pidx, d.word, d.face, k, (uint32_t)d.npts, cnew)
+                     : adr_entry<float, 2, NSM>(a, d.pidx, d.word, d.face, k, (uint32_t)d.npts, cnew);
   if (RED) {
     // per-block pore sums of every chemical (FP64), NSM slots per block
 #pragma unroll
@@ -247,7 +246,8 @@ extern "C" int prt_adr_batch_plan(const prt_adr_segment *segs, int nseg, void *d
   int nsm = 1;
   for (int s = 0; s < nseg; ++s) {
     const prt_adr_segment &g = segs[s];
-    if (!g.params || !g.flags || !g.u || !g.c_a || !g.c_b || g.c_a == g.c_b) {
+    if (!g.params || !g.c_a || !g.c_b || g.c_a == g.c_b ||
+        (g.n_pore > 0 && (!g.pore_idx || !g.word || !g.face))) {
       set_error("prt_adr_batch_plan: segment %d has null or aliased buffers", s);
       return PRT_E_ARG;
     }
@@ -255,16 +255,17 @@ extern "C" int prt_adr_batch_plan(const prt_adr_segment *segs, int nseg, void *d
     dim3 gr, bl;
     const int rc = adr_args_from_params<float>(g.params, g.nx, g.ny, g.nz, d[s].a, gr, bl);
     if (rc) return rc;
-    d[s].a.flags = g.flags;
-    d[s].a.u = g.u;
+    if (g.n_pore < 0 || g.n_pore > d[s].a.N || (unsigned long long)g.n_pore * 16ull >= (1ull << 32) ||
+        g.params->write_solid) {
+      set_error("prt_adr_batch_plan: segment %d: pore count out of range or write_solid set", s);
+      return PRT_E_ARG;
+    }
     d[s].ca = g.c_a;
     d[s].cb = g.c_b;
     d[s].pidx = g.pore_idx;
-    if (g.pore_idx && (g.n_pore < 0 || g.n_pore > d[s].a.N || g.params->write_solid)) {
-      set_error("prt_adr_batch_plan: segment %d pore list: count out of range or write_solid set", s);
-      return PRT_E_ARG;
-    }
-    d[s].npts = g.pore_idx ? (int)g.n_pore : (int)d[s].a.N;
+    d[s].word = g.word;
+    d[s].face = g.face;
+    d[s].npts = (int)g.n_pore;
     d[s].phase = g.phase & 1;
     d[s].nd = g.params->nd;
     d[s].blk0 = (int)total;

@@ -25,6 +25,31 @@ inline int launch_status(const char *what) {
   return e == hipSuccess ? PRT_OK : hip_fail(e, what);
 }
 
+// Division of a non-negative 31-bit index by a fixed divisor d >= 1 with a
+// multiply-high, an add and a shift (Granlund & Montgomery, "Division by
+// invariant integers using multiplication", PLDI 1994, round-up variant):
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
+// n / d = (mulhi(n, m) + n) >> l for 0 <= n < 2^31.
+struct FastDiv {
+  uint32_t m;
+  uint32_t l;
+  int d;
+};
+
+inline FastDiv make_fastdiv(int d) {
+  FastDiv f;
+  f.d = d < 1 ? 1 : d;
+  f.l = 0;
+  while ((1ull << f.l) < (unsigned long long)f.d) ++f.l;
+  f.m = (uint32_t)((((1ull << f.l) - (unsigned long long)f.d) << 32) / (unsigned long long)f.d + 1ull);
+  return f;
+}
+
+__device__ inline int fastdiv(int n, const FastDiv &f) {
+  const uint32_t u = (uint32_t)n;
+  return (int)((__umulhi(u, f.m) + u) >> f.l);
+}
+
 // Block-wide FP64 sum of one value per thread (blockDim.x * blockDim.y <=
 // 1024, a multiple of 64).  Wavefront reduction with shuffles, then LDS.
 __device__ inline double block_sum(double v, double *lds) {

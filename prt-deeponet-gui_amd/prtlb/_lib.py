@@ -63,8 +63,8 @@ class AdrSegment(ctypes.Structure):
     """Mirror of prt_adr_segment."""
     _fields_ = [("params", ctypes.POINTER(AdrParams)), ("nx", ctypes.c_int32),
                 ("ny", ctypes.c_int32), ("nz", ctypes.c_int32), ("phase", ctypes.c_int32),
-                ("flags", ctypes.c_void_p), ("u", ctypes.c_void_p), ("c_a", ctypes.c_void_p),
-                ("c_b", ctypes.c_void_p), ("pore_idx", ctypes.c_void_p), ("n_pore", ctypes.c_int64)]
+                ("pore_idx", ctypes.c_void_p), ("word", ctypes.c_void_p), ("face", ctypes.c_void_p),
+                ("n_pore", ctypes.c_int64), ("c_a", ctypes.c_void_p), ("c_b", ctypes.c_void_p)]
 
 
 _P = ctypes.c_void_p
@@ -86,6 +86,8 @@ _SIGS = {
     "prt_adr_step": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _P]),
     "prt_adr_step_f64": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _P]),
     "prt_adr_run": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _I, _P]),
+    "prt_adr_pore_table": (_I, [_I, _I, _I, _I, _P, _P, _P, _I64, _P, _P, _P]),
+    "prt_adr_pore_table_f64": (_I, [_I, _I, _I, _I, _P, _P, _P, _I64, _P, _P, _P]),
     "prt_adr_run_list": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _I64, _P, _P,
                               _I, _P]),
     "prt_adr_step_list_f64": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _I64, _P,

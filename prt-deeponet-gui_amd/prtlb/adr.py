@@ -257,6 +257,13 @@ class TransportSolver:
         self.latest = 0
         self.steps_done = 0
         self.pore_idx = torch.from_numpy(self.pore_idx_host).to(self.dev)
+        # pore table: per pore voxel its face bits / edge codes (uint32 kept
+        # in an int32 tensor) and face velocities, built once from flags and u
+        self.word = torch.empty(self.n_pore, dtype=torch.int32, device=self.dev)
+        self.face = torch.empty(2 * self.nd * self.n_pore, dtype=self.u.dtype, device=self.dev)
+        call("prt_adr_pore_table" if self.u.dtype == torch.float32 else "prt_adr_pore_table_f64",
+             self.nd, *self.dims, ptr(self.flags), ptr(self.u), ptr(self.pore_idx), self.n_pore,
+             ptr(self.word), ptr(self.face), self._stream())
         self._plan = None
         self.means_scratch = None
         self.use_graph = self.n < (1 << 20)
@@ -289,8 +296,8 @@ class TransportSolver:
         """`steps` float32 steps a -> b -> ... (pore-list launch by default)."""
         pp = ctypes.byref(self.params)
         if self.pore_list:
-            call("prt_adr_run_list", pp, *self.dims, ptr(self.flags), ptr(self.u),
-                 ptr(self.pore_idx), self.n_pore, ptr(a), ptr(b), int(steps), stream_ptr(self.dev))
+            call("prt_adr_run_list", pp, *self.dims, ptr(self.pore_idx), ptr(self.word),
+                 ptr(self.face), self.n_pore, ptr(a), ptr(b), int(steps), stream_ptr(self.dev))
         else:
             call("prt_adr_run", pp, *self.dims, ptr(self.flags), ptr(self.u), ptr(a), ptr(b),
                  int(steps), stream_ptr(self.dev))
@@ -325,8 +332,8 @@ class TransportSolver:
         else:
             for s in range(steps):
                 src, dst = (a, b) if s % 2 == 0 else (b, a)
-                call("prt_adr_step_list_f64", pp, *self.dims, ptr(self.flags), ptr(self.u),
-                     ptr(self.pore_idx), self.n_pore, ptr(src), ptr(dst), self._stream())
+                call("prt_adr_step_list_f64", pp, *self.dims, ptr(self.pore_idx), ptr(self.word),
+                     ptr(self.face), self.n_pore, ptr(src), ptr(dst), self._stream())
         if steps % 2:
             self.latest = 1 - self.latest
         self.steps_done += steps
