@@ -182,20 +182,6 @@ __device__ inline void adr_update(const AdrArgs<R> &a, uint32_t n, const VoxGeom
   const int ns = a.ns;
   const uint32_t w = g.w;
   const int bf_self = (w >> 18) & 1;
-  // the inlet pin (3D/tools/prtlb_3d.py:1398-1400) applied as the value is
-  // read: a voxel on x = 0 that is read at all is a pore voxel
-  const uint32_t xlo = (w >> 19) & 3u;
-  const bool x_is0 = xlo == 2, x_le1 = xlo != 0, x_le2 = x_le1 || (w >> 31);
-  bool onx0[ND][4];
-#pragma unroll
-  for (int ax = 0; ax < ND; ++ax)
-#pragma unroll
-    for (int k = 0; k < 4; ++k) onx0[ax][k] = x_is0;
-  onx0[0][0] = x_le2;
-  onx0[0][1] = x_le1;
-  onx0[0][2] = a.X == 1;
-  onx0[0][3] = a.X == 1;
-
   // every neighbour value is loaded up front and unconditionally (indices
   // are clamped, so always in bounds); the face rules pick among them
   R c[NSM], tr[NSM], cn[NSM][ND][4];
@@ -205,16 +191,39 @@ __device__ inline void adr_update(const AdrArgs<R> &a, uint32_t n, const VoxGeom
     c[j] = R(0);
     if (j >= ns) continue;
     const R *cj = a.cin + (size_t)j * a.N;
-    const R v = ld32(cj, n);
-    c[j] = (x_is0 && a.pinned[j]) ? a.fed[j] : v;
+    c[j] = ld32(cj, n);
     if (!a.mobile[j]) continue;
 #pragma unroll
     for (int ax = 0; ax < ND; ++ax)
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const R v2 = ld32(cj, g.nidx[ax][k]);
-        cn[j][ax][k] = (onx0[ax][k] && a.pinned[j]) ? a.fed[j] : v2;
-      }
+      for (int k = 0; k < 4; ++k) cn[j][ax][k] = ld32(cj, g.nidx[ax][k]);
+  }
+  // the inlet pin (3D/tools/prtlb_3d.py:1398-1400) applied as the values are
+  // read: a voxel on x = 0 that is read at all is a pore voxel.  Only voxels
+  // with x <= 2 (or a one-plane grid) can see x = 0.
+  const uint32_t xlo = (w >> 19) & 3u;
+  const bool x_is0 = xlo == 2, x_le1 = xlo != 0, x_le2 = x_le1 || (w >> 31);
+  if (x_le2 || a.X == 1) {
+    bool onx0[ND][4];
+#pragma unroll
+    for (int ax = 0; ax < ND; ++ax)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) onx0[ax][k] = x_is0;
+    onx0[0][0] = x_le2;
+    onx0[0][1] = x_le1;
+    onx0[0][2] = a.X == 1;
+    onx0[0][3] = a.X == 1;
+#pragma unroll
+    for (int j = 0; j < NSM; ++j) {
+      if (j >= ns || !a.pinned[j]) continue;
+      if (x_is0) c[j] = a.fed[j];
+      if (!a.mobile[j]) continue;
+#pragma unroll
+      for (int ax = 0; ax < ND; ++ax)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (onx0[ax][k]) cn[j][ax][k] = a.fed[j];
+    }
   }
 
   R divu = R(0);
@@ -241,8 +250,11 @@ __device__ inline void adr_update(const AdrArgs<R> &a, uint32_t n, const VoxGeom
       const R d_hi = o_hi ? (bp1 ? t1 : t0) : R(0);
       const R dif = d_hi * (Cp1 - C) + d_lo * (Cm1 - C);
       tr[j] = (ax == 0) ? R(0) + dif : tr[j] + dif;
-      const R fh = u_hi >= R(0) ? muscl(C, Cm1, Cp1) : muscl(Cp1, Cp2, C);
-      const R flo = u_lo >= R(0) ? muscl(Cm1, Cm2, C) : muscl(C, Cp1, Cm1);
+      // upwind face values: the upwind side picks the three muscl operands,
+      // so one evaluation per face (u >= 0 is false for NaN, as in np.where)
+      const bool uh = u_hi >= R(0), ul = u_lo >= R(0);
+      const R fh = muscl(uh ? C : Cp1, uh ? Cm1 : Cp2, uh ? Cp1 : C);
+      const R flo = muscl(ul ? Cm1 : C, ul ? Cm2 : Cp1, ul ? C : Cm1);
       tr[j] = tr[j] - (u_hi * fh - u_lo * flo);
     }
   }

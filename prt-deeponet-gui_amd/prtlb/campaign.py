@@ -118,40 +118,60 @@ def _adr_kw(spec, kinetics):
 
 
 def run_sequential(specs, geoms, storage="fp64", tau=0.8, tol=1e-6, nit=30000,
-                   kinetics="sorption_irreversible", n_t=5, adr_steps=None, max_steps=100000):
-    """One solve after another (the reference's own schedule, on the GPU)."""
+                   kinetics="sorption_irreversible", n_t=5, adr_steps=None, max_steps=100000,
+                   adr_precision="fp32", progress=None):
+    """One solve after another (the reference's own schedule, on the GPU).
+
+    Returns (results by key, seconds) with seconds = {"total", "flow",
+    "transport"}: the Stokes solves (probe + real, every rock) and the
+    transport solves timed as stages, as run_batched does.  progress, if
+    given, is called with a short text after every finished solve."""
     res = {}
     t0 = time.perf_counter()
+    flows = {}
     for sp in specs:
         g = geoms[sp["key"]]
         fn = stokes_d2q9 if g.ndim == 2 else stokes_d3q19
         probe = fn(g, nit=nit, force=PROBE_FORCE, tau_lb=tau, tol=tol, storage=storage)
         force = lowered_force(probe, 2e-6)
         info = {}
-        v = fn(g, nit=nit, force=force, tau_lb=tau, tol=tol, info=info, storage=storage)
+        flows[sp["key"]] = fn(g, nit=nit, force=force, tau_lb=tau, tol=tol, info=info,
+                              storage=storage)
+        res[sp["key"]] = dict(permeability=info["permeability"], flow_iters=info["flow_iters"],
+                              force=force)
+        if progress:
+            progress("flow %s done (%d its)" % (sp["key"], info["flow_iters"]))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for sp in specs:
+        g = geoms[sp["key"]]
         ainfo = {}
         kw = _adr_kw(sp, kinetics)
-        out, tn = solve_adr(g, v, sp["pe"], kw.pop("da"), n_t, steps=adr_steps,
-                            max_steps=max_steps, info=ainfo, **kw)
-        res[sp["key"]] = dict(permeability=info["permeability"], flow_iters=info["flow_iters"],
-                              force=force, adr_steps=ainfo["steps"],
+        out, tn = solve_adr(g, flows[sp["key"]], sp["pe"], kw.pop("da"), n_t, steps=adr_steps,
+                            max_steps=max_steps, info=ainfo, precision=adr_precision, **kw)
+        res[sp["key"]].update(adr_steps=ainfo["steps"], converged=ainfo["converged"],
                               outlet=np.asarray(ainfo["outlet_concentration"]).tolist(),
                               out_last=out[-1])
+        if progress:
+            progress("transport %s done (%d steps)" % (sp["key"], ainfo["steps"]))
     torch.cuda.synchronize()
-    return res, time.perf_counter() - t0
+    t2 = time.perf_counter()
+    return res, dict(total=t2 - t0, flow=t1 - t0, transport=t2 - t1)
 
 
 def run_batched(specs, geoms, storage="fp32", tau=0.8, tol=1e-6, nit=30000,
                 kinetics="sorption_irreversible", n_t=5, adr_steps=None, max_steps=100000,
-                hbm_budget=200e9):
+                hbm_budget=200e9, progress=None):
     """Every Stokes solve in one LbmBatch per lattice, every transport solve
-    in one AdrBatch; members admitted while they fit the HBM budget."""
+    in one AdrBatch; members admitted while they fit the HBM budget.
+    Returns (results by key, seconds) as run_sequential."""
     res = {sp["key"]: {} for sp in specs}
     t0 = time.perf_counter()
     nd = len(specs[0]["shape"])
     q = 9 if nd == 2 else 19
     # ---- Stokes: probe stage then real stage, one batch
     lb = LbmBatch(q, storage=storage)
+    t_note = time.perf_counter()
     pending = [(sp["key"], "probe") for sp in specs]
     velocities = {}
     while pending or len(lb):
@@ -164,7 +184,11 @@ def run_batched(specs, geoms, storage="fp32", tau=0.8, tol=1e-6, nit=30000,
             force = PROBE_FORCE if stage == "probe" else res[key]["force"]
             m = lb.add((key, stage), g, force=force, tau_lb=tau, tol=tol, nit=nit)
             m.stage = stage
-        for m in lb.tick():
+        finished = lb.tick()
+        if progress and time.perf_counter() - t_note > 20.0:
+            t_note = time.perf_counter()
+            progress("flow batch: %d members running, %d waiting" % (len(lb), len(pending)))
+        for m in finished:
             key, stage = m.key
             info = {}
             v = lb.finish(m, info).cpu().numpy()
@@ -175,6 +199,10 @@ def run_batched(specs, geoms, storage="fp32", tau=0.8, tol=1e-6, nit=30000,
                 res[key]["permeability"] = info["permeability"]
                 res[key]["flow_iters"] = m.conv.iters
                 velocities[key] = v
+                if progress:
+                    progress("flow %s done (%d its)" % (key, m.conv.iters))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
     # ---- transport: one batch
     ab = AdrBatch()
     queue = list(specs)
@@ -188,12 +216,19 @@ def run_batched(specs, geoms, storage="fp32", tau=0.8, tol=1e-6, nit=30000,
             queue.pop(0)
             ab.add(sp["key"], g, velocities[sp["key"]], sp["pe"], kw.pop("da"), n_t,
                    steps=adr_steps, max_steps=max_steps, **kw)
-        for m in ab.advance():
+        finished = ab.advance()
+        if progress and time.perf_counter() - t_note > 20.0:
+            t_note = time.perf_counter()
+            progress("transport batch: %d members running, %d waiting" % (len(ab), len(queue)))
+        for m in finished:
             ainfo = {}
             out, tn = ab.finish(m, ainfo)
             r = res[m.key]
-            r.update(adr_steps=ainfo["steps"],
+            r.update(adr_steps=ainfo["steps"], converged=ainfo["converged"],
                      outlet=np.asarray(ainfo["outlet_concentration"]).tolist(),
                      out_last=out[-1])
+            if progress:
+                progress("transport %s done (%d steps)" % (m.key, ainfo["steps"]))
     torch.cuda.synchronize()
-    return res, time.perf_counter() - t0
+    t2 = time.perf_counter()
+    return res, dict(total=t2 - t0, flow=t1 - t0, transport=t2 - t1)
