@@ -246,7 +246,8 @@ def serve_study(n_req=48, rate=4.0, seed=0, budget=200e9, max_steps=100000):
     reqs = serve.poisson_requests(n_req, rate=rate, seed=seed)
     log("serve geometry %.1f s" % (time.perf_counter() - t0))
     t0 = time.perf_counter()
-    done = serve.Server(hbm_budget=budget, max_steps=max_steps).run(reqs, realtime=True)
+    done = serve.Server(hbm_budget=budget, max_steps=max_steps).run(reqs, realtime=True,
+                                                                     progress=log)
     wall = time.perf_counter() - t0
     out = serve.latency_summary(done, wall)
     out.update(rate_per_s=rate, hbm_budget_GB=budget / 1e9,

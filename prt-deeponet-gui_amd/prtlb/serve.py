@@ -110,14 +110,21 @@ class Server:
     def idle(self):
         return not (self.waiting or any(len(b) for b in self.lb.values()) or len(self.ab))
 
-    def run(self, requests, realtime=True):
+    def run(self, requests, realtime=True, progress=None):
         """Serve `requests` (arrival times in seconds from now when realtime,
         else all at once) until every one is done.  Returns them, with
-        latency = done - arrival."""
+        latency = done - arrival.  progress, if given, is called with a short
+        status text every 20 s of serving."""
         t0 = time.perf_counter()
+        t_note = t0
         for r in sorted(requests, key=lambda r: r.arrival):
             self.submit(r)
         while not self.idle():
+            if progress and time.perf_counter() - t_note > 20.0:
+                t_note = time.perf_counter()
+                progress("serve %.0f s: %d done, %d in flow, %d in transport, %d waiting, %.1f GB held"
+                         % (t_note - t0, len(self.done), sum(len(b) for b in self.lb.values()),
+                            len(self.ab), len(self.waiting), self.in_use() / 1e9))
             now = (time.perf_counter() - t0) if realtime else float("inf")
             busy = any(len(b) for b in self.lb.values()) or len(self.ab)
             if not busy and self.waiting and realtime and self.waiting[0].arrival > now:
