@@ -318,6 +318,57 @@ int prt_coat_with_biofilm(uint8_t *g, int nx, int ny, int nz, int pore_code,
                           uint8_t *work_b, int *scratch1, int *marked_host,
                           void *stream);
 
+/* Geodesic distance from the inlet through the pore, the relaxation of
+ * 3D/tools/build_practice_dataset.py:53-72 (geodesic) pass for pass: float32,
+ * solid -> 0 in out, unreached pore keeps 1e9.  d_a, d_b: n floats of work;
+ * scratch1: one int.  Runs at most max_sweeps sweeps (the reference uses
+ * 4 nx) and reports the sweeps run (a multiple of 8 or max_sweeps).
+ * Synchronises the stream. */
+int prt_geodesic(const uint8_t *g, int nx, int ny, int nz, int pore_code,
+                 float *d_a, float *d_b, int *scratch1, int max_sweeps,
+                 float *out, int *sweeps_host, void *stream);
+
+/* Euclidean distance of every pore voxel to the nearest non-pore voxel,
+ * float32, 0 on non-pore: scipy.ndimage.distance_transform_edt(g == PORE)
+ * as 3D/tools/build_dataset_3d.py:659 stores it.  work_a, work_b: n int32.
+ * The grid must hold at least one non-pore voxel (the caller checks); edges
+ * above 16384 are refused. */
+int prt_edt(const uint8_t *g, int nx, int ny, int nz, int pore_code,
+            int32_t *work_a, int32_t *work_b, float *out, void *stream);
+
+/* ------------------------------------------------ check_physics passes */
+/* The data passes of check_physics (3D/tools/prtlb_3d.py:1632-1843) over
+ * device snapshots out (n_t, ns, n_vox) float32 with a pore byte mask
+ * (1 = pore).  Each writes one partial per block (`blocks` from
+ * prt_check_blocks); the caller folds the partials. */
+int prt_check_blocks(int64_t n);
+/* counts[2b], counts[2b+1]: NaNs and infinities of x[0 .. n) */
+int prt_check_nonfinite(const float *x, int64_t n, int blocks, int64_t *counts,
+                        void *stream);
+/* mx/mn[(t*ns + j)*blocks + b]: pore max / min of out[t, j] (-inf / +inf
+ * for a block without pore) */
+int prt_check_pore_range(const float *out, int n_t, int ns, int64_t n_vox,
+                         const uint8_t *pore, int blocks, float *mx, float *mn,
+                         void *stream);
+/* part[(p*n_t + t)*blocks + b]: max of chemical 0 of snapshot t over the x
+ * plane p (0: x = nx/2, 1: x = nx-1), plane = voxels per x plane */
+int prt_check_plane_max(const float *out, int n_t, int ns, int nx, int64_t plane,
+                        int blocks, float *part, void *stream);
+/* counts[2b], counts[2b+1]: pore triples along x and sign flips among them
+ * in the field f (the last snapshot of chemical 0) */
+int prt_check_checker(const float *f, const uint8_t *pore, int nx, int64_t plane,
+                      int blocks, int64_t *counts, void *stream);
+/* mom[((j-1)*blocks + b)*5 + q], j = 1 .. ns-1: FP64 sums over (snapshot,
+ * pore voxel) of a0, aj, (a0-m0)^2, (aj-mj)^2, (a0-m0)(aj-mj), m = mean
+ * (device, ns doubles) or 0 when mean is NULL */
+int prt_check_moments(const float *out, int n_t, int ns, const uint8_t *pore,
+                      int64_t n_vox, int blocks, const double *mean,
+                      double *mom, void *stream);
+/* part[b]: max |aj - sc a0| in float32 over (snapshot, pore voxel) */
+int prt_check_copy_residual(const float *out, int n_t, int ns, int j, float sc,
+                            const uint8_t *pore, int64_t n_vox, int blocks,
+                            float *part, void *stream);
+
 /* --------------------------------------------------------- measurement */
 
 /* Plain device-to-device copy of `bytes` (multiple of 16, 16-byte aligned

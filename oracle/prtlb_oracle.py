@@ -370,6 +370,71 @@ def coat_with_biofilm(g, code, layers=1):
 # ---------------------------------------------------------------------------
 # BLOCK 7-9: transport (3D/tools/prtlb_3d.py:650-1595)
 # ---------------------------------------------------------------------------
+def geodesic(g, max_sweeps=None):
+    """Geodesic distance from the inlet through the pore, float32; the
+    relaxation of 3D/tools/build_practice_dataset.py:53-72 (NaN in solid, 1e9
+    in unreached pore, x = 0 pore at 0; per sweep six Jacobi passes, one per
+    axis and shift direction, each taking min(d, d(upstream) + 1) with solid
+    and out-of-grid upstream read as 1e9; stop after a sweep that changes
+    nothing or after 4 nx sweeps; solid written as 0).  Returns (d, sweeps
+    run).  Written with index slices instead of np.roll."""
+    g = np.asarray(g)
+    if max_sweeps is None:
+        max_sweeps = 4 * g.shape[0]
+    pore = g == PORE
+    inf = np.float32(1e9)
+    d = np.where(pore, inf, np.float32(np.nan)).astype(np.float32)
+    d[0][pore[0]] = 0.0
+    sweeps = 0
+    for _ in range(max_sweeps):
+        sweeps += 1
+        prev = d.copy()
+        for ax in range(g.ndim):
+            for sh in (1, -1):
+                nb = np.full_like(d, inf)
+                dst = [slice(None)] * g.ndim
+                src = [slice(None)] * g.ndim
+                if sh > 0:
+                    dst[ax], src[ax] = slice(1, None), slice(0, -1)
+                else:
+                    dst[ax], src[ax] = slice(0, -1), slice(1, None)
+                up = d[tuple(src)]
+                nb[tuple(dst)] = np.where(np.isnan(up), inf, up)
+                d = np.where(pore, np.fmin(d, nb + np.float32(1.0)), np.float32(np.nan))
+        change = np.where(pore, np.abs(d - prev), 0.0)
+        if change.max(initial=0.0) < 1e-6:
+            break
+    return np.where(pore, d, 0.0).astype(np.float32), sweeps
+
+
+def bfs_distance(g):
+    """Face-connected breadth-first distance from the x = 0 pore (1e9 where
+    unreached, 0 in solid): what the relaxation converges to.  A different
+    algorithm, used to pin the relaxation."""
+    from collections import deque
+    g = np.asarray(g)
+    pore = g == PORE
+    d = np.full(g.shape, np.float32(1e9), np.float32)
+    q = deque()
+    for idx in zip(*np.nonzero(pore[0])):
+        full = (0,) + tuple(idx)
+        d[full] = 0.0
+        q.append(full)
+    while q:
+        c = q.popleft()
+        for ax in range(g.ndim):
+            for s in (-1, 1):
+                nb = list(c)
+                nb[ax] += s
+                if not (0 <= nb[ax] < g.shape[ax]):
+                    continue
+                nb = tuple(nb)
+                if pore[nb] and d[nb] > d[c] + 1:
+                    d[nb] = d[c] + 1
+                    q.append(nb)
+    return np.where(pore, d, 0.0).astype(np.float32)
+
+
 def muscl(up, upup, down):
     """Minmod-limited face value; 3D/tools/prtlb_3d.py:650-705."""
     d1 = down - up

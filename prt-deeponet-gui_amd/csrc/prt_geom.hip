@@ -17,7 +17,23 @@
 // a pore voxel on x = 0 are found by repeated face dilation restricted to
 // the pore until nothing changes; the rest of the pore becomes SOLID.
 // Coating (:567-593): `layers` face dilations of the non-pore, no wrap.
+//
+// Geodesic distance from the inlet (3D/tools/build_practice_dataset.py:
+// 53-72): the reference's relaxation, pass for pass -- per sweep, for each
+// axis and each shift direction in turn, every pore voxel takes
+// min(d, d(upstream neighbour) + 1) with solid and out-of-grid neighbours
+// read as 1e9, all in float32; sweeps stop after the first one that changes
+// nothing, or after 4 nx sweeps.  Each pass is a Jacobi update from the
+// previous pass's array, as the reference's whole-array roll is.
+//
+// Euclidean distance transform of the pore (scipy.ndimage.
+// distance_transform_edt(g == PORE), used at 3D/tools/build_dataset_3d.py:659):
+// exact squared distances to the nearest non-pore voxel by three separable
+// minimisations (z, then y, then x) in integers, then sqrt in FP64 rounded
+// to float32, which is what the reference stores.
 #include <hip/hip_runtime.h>
+
+#include <cmath>
 
 #include "prt_common.h"
 
@@ -142,6 +158,102 @@ __global__ __launch_bounds__(256) void coat_apply_kernel(uint8_t *g, uint8_t por
     g[n] = code;
     atomicAdd(marked, 1);
   }
+}
+
+// ---- geodesic relaxation
+constexpr float kGeoInf = 1e9f;
+
+__global__ __launch_bounds__(256) void geo_init_kernel(const uint8_t *g, uint8_t pore, long long plane, long long N,
+                                                       float *d) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  d[n] = g[n] == pore ? (n < plane ? 0.0f : kGeoInf) : __builtin_nanf("");
+}
+
+// One (axis, shift) pass: dout = min(din, din[upstream] + 1) on pore voxels,
+// the upstream voxel being the one at -sh along the axis; *changed is set
+// when a value drops.
+__global__ __launch_bounds__(256) void geo_pass_kernel(const uint8_t *g, uint8_t pore, int X, int Y, int Z, int ax,
+                                                       int sh, const float *din, float *dout, int *changed) {
+#pragma clang fp contract(off)
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long N = (long long)X * Y * Z;
+  if (n >= N) return;
+  if (g[n] != pore) {
+    dout[n] = __builtin_nanf("");
+    return;
+  }
+  const long long yz = (long long)Y * Z;
+  const int x = (int)(n / yz);
+  const int r = (int)(n - x * yz);
+  const int y = r / Z, z = r - (r / Z) * Z;
+  const int p = ax == 0 ? x : (ax == 1 ? y : z);
+  const int ext = ax == 0 ? X : (ax == 1 ? Y : Z);
+  const long long stride = ax == 0 ? yz : (ax == 1 ? (long long)Z : 1);
+  float nb = kGeoInf;
+  const int q = p - sh;
+  if (q >= 0 && q < ext) {
+    const float v = din[n - (long long)sh * stride];
+    nb = v != v ? kGeoInf : v;
+  }
+  const float d = din[n];
+  const float cand = nb + 1.0f;
+  const float out = cand < d ? cand : d;
+  dout[n] = out;
+  if (out != d) *changed = 1;
+}
+
+__global__ __launch_bounds__(256) void geo_finish_kernel(const float *d, long long N, float *out) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < N) out[n] = d[n] != d[n] ? 0.0f : d[n];
+}
+
+// ---- exact Euclidean distance transform (squared, int32)
+constexpr int kEdtInf = 1 << 30;
+
+// D[n] = min over z' of the row: non-pore at z' ? (z - z')^2 : inf.
+__global__ __launch_bounds__(256) void edt_z_kernel(const uint8_t *g, uint8_t pore, int Z, long long N, int *D) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const long long row = n / Z;
+  const int z = (int)(n - row * Z);
+  const uint8_t *gr = g + row * Z;
+  int best = kEdtInf;
+  for (int q = 0; q < Z; ++q)
+    if (gr[q] != pore) {
+      const int d = z - q;
+      best = d * d < best ? d * d : best;
+    }
+  D[n] = best;
+}
+
+// Dout[n] = min over the axis line through n of Din[line k] + (p - k)^2.
+__global__ __launch_bounds__(256) void edt_line_kernel(const int *Din, int X, int Y, int Z, int ax, int *Dout) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long N = (long long)X * Y * Z;
+  if (n >= N) return;
+  const long long yz = (long long)Y * Z;
+  const int x = (int)(n / yz);
+  const int r = (int)(n - x * yz);
+  const int y = r / Z;
+  const int p = ax == 0 ? x : y;
+  const int ext = ax == 0 ? X : Y;
+  const long long stride = ax == 0 ? yz : (long long)Z;
+  const int *line = Din + (n - (long long)p * stride);
+  int best = kEdtInf;
+  for (int k = 0; k < ext; ++k) {
+    const int v = line[(long long)k * stride];
+    if (v >= kEdtInf) continue;
+    const int d = p - k;
+    const int c = v + d * d;
+    best = c < best ? c : best;
+  }
+  Dout[n] = best;
+}
+
+__global__ __launch_bounds__(256) void edt_sqrt_kernel(const int *D, long long N, float *out) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < N) out[n] = (float)sqrt((double)D[n]);
 }
 
 static unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
@@ -275,4 +387,65 @@ extern "C" int prt_coat_with_biofilm(uint8_t *g, int nx, int ny, int nz, int por
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hip_fail(e, "prt_coat_with_biofilm count copy");
   return PRT_OK;
+}
+
+extern "C" int prt_geodesic(const uint8_t *g, int nx, int ny, int nz, int pore_code, float *d_a, float *d_b,
+                            int *scratch1, int max_sweeps, float *out, int *sweeps_host, void *stream) {
+  int X, Y, Z;
+  int rc = dims3(nx, ny, nz, X, Y, Z);
+  if (rc) return rc;
+  if (!g || !d_a || !d_b || !scratch1 || !out || max_sweeps < 0) {
+    set_error("prt_geodesic: bad arguments");
+    return PRT_E_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const long long N = (long long)X * Y * Z, plane = (long long)Y * Z;
+  geo_init_kernel<<<nblk(N), 256, 0, st>>>(g, (uint8_t)pore_code, plane, N, d_a);
+  if ((rc = launch_status("geodesic init"))) return rc;
+  float *src = d_a, *dst = d_b;
+  int done = 0;
+  // sweeps in chunks between host checks; a chunk that changes nothing
+  // means the relaxation had stopped, and sweeps past that are identities
+  while (done < max_sweeps) {
+    const int chunk = max_sweeps - done < 8 ? max_sweeps - done : 8;
+    hipError_t e = hipMemsetAsync(scratch1, 0, sizeof(int), st);
+    if (e != hipSuccess) return hip_fail(e, "prt_geodesic memset");
+    for (int k = 0; k < chunk; ++k)
+      for (int ax = 0; ax < 3; ++ax)
+        for (int sh = 1; sh >= -1; sh -= 2) {
+          geo_pass_kernel<<<nblk(N), 256, 0, st>>>(g, (uint8_t)pore_code, X, Y, Z, ax, sh, src, dst, scratch1);
+          float *t = src;
+          src = dst;
+          dst = t;
+        }
+    if ((rc = launch_status("geodesic pass"))) return rc;
+    done += chunk;
+    int h = 0;
+    e = hipMemcpyAsync(&h, scratch1, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "prt_geodesic flag copy");
+    if (!h) break;
+  }
+  geo_finish_kernel<<<nblk(N), 256, 0, st>>>(src, N, out);
+  if ((rc = launch_status("geodesic finish"))) return rc;
+  if (sweeps_host) *sweeps_host = done;
+  return PRT_OK;
+}
+
+extern "C" int prt_edt(const uint8_t *g, int nx, int ny, int nz, int pore_code, int32_t *work_a, int32_t *work_b,
+                       float *out, void *stream) {
+  int X, Y, Z;
+  int rc = dims3(nx, ny, nz, X, Y, Z);
+  if (rc) return rc;
+  if (!g || !work_a || !work_b || !out || X > 16384 || Y > 16384 || Z > 16384) {
+    set_error("prt_edt: bad arguments (or an edge above 16384)");
+    return PRT_E_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const long long N = (long long)X * Y * Z;
+  edt_z_kernel<<<nblk(N), 256, 0, st>>>(g, (uint8_t)pore_code, Z, N, work_a);
+  edt_line_kernel<<<nblk(N), 256, 0, st>>>(work_a, X, Y, Z, 1, work_b);
+  edt_line_kernel<<<nblk(N), 256, 0, st>>>(work_b, X, Y, Z, 0, work_a);
+  edt_sqrt_kernel<<<nblk(N), 256, 0, st>>>(work_a, N, out);
+  return launch_status("prt_edt launch");
 }

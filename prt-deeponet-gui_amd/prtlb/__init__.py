@@ -12,9 +12,10 @@ from ._lib import PrtError
 from .adr import TransportSolver, solve_adr
 from .chemistry import Chemistry, default_chemistry
 from .geometry import (PORE, SOLID, WALL, assert_finite_distance,
-                       coat_with_biofilm, disk_pack, keep_inlet_connected,
-                       sphere_pack)
-from .physics import check_physics, outlet_concentration, permeability
+                       coat_with_biofilm, disk_pack, edt_device, geodesic_device,
+                       keep_inlet_connected, sphere_pack)
+from .physics import (check_physics, check_physics_device, outlet_concentration,
+                      permeability)
 from .stokes import (FlowConvergence, StokesSolver, plane_flux, stokes_d2q9,
                      stokes_d3q19, trt_minus)
 
@@ -33,5 +34,6 @@ __all__ = [
     "keep_inlet_connected", "coat_with_biofilm", "assert_finite_distance",
     "Chemistry", "default_chemistry", "permeability", "outlet_concentration",
     "StokesSolver", "TransportSolver", "FlowConvergence", "trt_minus", "plane_flux",
-    "disk_pack", "sphere_pack", "PrtError",
+    "disk_pack", "sphere_pack", "PrtError", "check_physics_device", "geodesic_device",
+    "edt_device",
 ]

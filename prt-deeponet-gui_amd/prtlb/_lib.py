@@ -86,6 +86,15 @@ _SIGS = {
     "prt_adr_step": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _P]),
     "prt_adr_step_f64": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _P]),
     "prt_adr_run": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _I, _P]),
+    "prt_geodesic": (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _I, _P, _P, _P]),
+    "prt_edt": (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P]),
+    "prt_check_blocks": (_I, [_I64]),
+    "prt_check_nonfinite": (_I, [_P, _I64, _I, _P, _P]),
+    "prt_check_pore_range": (_I, [_P, _I, _I, _I64, _P, _I, _P, _P, _P]),
+    "prt_check_plane_max": (_I, [_P, _I, _I, _I, _I64, _I, _P, _P]),
+    "prt_check_checker": (_I, [_P, _P, _I, _I64, _I, _P, _P]),
+    "prt_check_moments": (_I, [_P, _I, _I, _P, _I64, _I, _P, _P, _P]),
+    "prt_check_copy_residual": (_I, [_P, _I, _I, _I, ctypes.c_float, _P, _I64, _I, _P, _P]),
     "prt_adr_pore_table": (_I, [_I, _I, _I, _I, _P, _P, _P, _I64, _P, _P, _P]),
     "prt_adr_pore_table_f64": (_I, [_I, _I, _I, _I, _P, _P, _P, _I64, _P, _P, _P]),
     "prt_adr_run_list": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _I64, _P, _P,

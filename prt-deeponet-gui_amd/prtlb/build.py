@@ -25,6 +25,7 @@ SOURCES = {
     "prt_reduce.hip": ["-ffp-contract=off"],
     "prt_batch.hip": [],
     "prt_geom.hip": ["-ffp-contract=off"],
+    "prt_check.hip": ["-ffp-contract=off"],
 }
 
 

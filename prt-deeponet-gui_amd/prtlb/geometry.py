@@ -239,3 +239,58 @@ def sphere_pack(shape=(128, 128, 128), radius=(4.0, 8.0), porosity=0.35, seed=0,
     """Random overlapping solid spheres (3D), radii U(lo, hi).
     device: build it on that GPU (bit-identical to the host build)."""
     return _pack(shape, radius, porosity, seed, connect, device)
+
+
+def _device_grid(g, device=None):
+    import torch
+    from .device import device as _dev
+    if isinstance(g, torch.Tensor):
+        t = g.to(torch.uint8).contiguous()
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(np.asarray(g), np.uint8)).to(_dev(device))
+    shape = tuple(t.shape)
+    if len(shape) not in (2, 3):
+        raise ValueError("grid must be 2D or 3D")
+    dims = (shape[0], shape[1], 1 if len(shape) == 2 else shape[2])
+    return t, shape, dims
+
+
+def geodesic_device(g, max_sweeps=None, device=None):
+    """Geodesic distance from the inlet through the pore on the GPU: the
+    relaxation the reference's dataset builders store as geom/gdf
+    (3D/tools/build_practice_dataset.py:53-72), float32, bit-identical.
+    Returns (distance tensor shaped like g, sweeps run)."""
+    import torch
+    from ._lib import call
+    from .device import ptr, stream_ptr
+    t, shape, dims = _device_grid(g, device)
+    n = t.numel()
+    if max_sweeps is None:
+        max_sweeps = 4 * shape[0]
+    da = torch.empty(n, dtype=torch.float32, device=t.device)
+    db = torch.empty_like(da)
+    out = torch.empty_like(da)
+    flag = torch.zeros(1, dtype=torch.int32, device=t.device)
+    sw = ctypes.c_int()
+    call("prt_geodesic", ptr(t), *dims, PORE, ptr(da), ptr(db), ptr(flag), int(max_sweeps), ptr(out),
+         ctypes.byref(sw), stream_ptr(t.device))
+    return out.reshape(shape), int(sw.value)
+
+
+def edt_device(g, device=None):
+    """Euclidean distance of each pore voxel to the nearest non-pore voxel on
+    the GPU, float32: scipy.ndimage.distance_transform_edt(g == PORE) as the
+    reference's dataset builders store it (3D/tools/build_dataset_3d.py:659),
+    bit-identical.  The grid must hold a non-pore voxel."""
+    import torch
+    from ._lib import call
+    from .device import ptr, stream_ptr
+    t, shape, dims = _device_grid(g, device)
+    if not bool((t != PORE).any().item()):
+        raise ValueError("edt_device: the grid has no non-pore voxel to measure from")
+    n = t.numel()
+    wa = torch.empty(n, dtype=torch.int32, device=t.device)
+    wb = torch.empty_like(wa)
+    out = torch.empty(n, dtype=torch.float32, device=t.device)
+    call("prt_edt", ptr(t), *dims, PORE, ptr(wa), ptr(wb), ptr(out), stream_ptr(t.device))
+    return out.reshape(shape)

@@ -94,3 +94,23 @@ def test_check_physics_golden(case):
     assert len(got) == len(ref)
     for kw in KEYWORDS[case]:
         assert any(kw in s for s in ref) and any(kw in s for s in got)
+
+
+def test_oracle_geodesic_pinned_by_bfs_and_closed_forms():
+    """The relaxation geodesic (3D/tools/build_practice_dataset.py:53-72)
+    equals breadth-first distances once it has converged, is x along a
+    straight channel, and moves one voxel per pass (Jacobi, not in-place)."""
+    rng = np.random.default_rng(3)
+    for shape, p in [((12, 9, 7), 0.3), ((20, 6, 5), 0.45), ((15, 15, 1), 0.35)]:
+        g = np.where(rng.random(shape) < p, 0, 2).astype(np.uint8)
+        d, sw = O.geodesic(g)
+        assert sw < 4 * shape[0]
+        assert np.array_equal(d, O.bfs_distance(g))
+    g = np.full((10, 3, 3), 2, np.uint8)
+    d, _ = O.geodesic(g)
+    assert np.array_equal(d, np.broadcast_to(np.arange(10, dtype=np.float32)[:, None, None], g.shape))
+    d1, sw1 = O.geodesic(g, max_sweeps=1)
+    assert sw1 == 1 and d1[1, 0, 0] == 1.0 and d1[2, 0, 0] == np.float32(1e9)
+    g[:, 1, 1] = 0                                 # solid core: distance 0 there
+    d, _ = O.geodesic(g)
+    assert (d[:, 1, 1] == 0).all()
