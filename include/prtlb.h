@@ -51,6 +51,8 @@ extern "C" {
   (((bz_log2) << 8) | ((by_log2) << 12) | (((npt) - 1) << 16))
 /* nontemporal population loads (dense mode, one node per thread) */
 #define PRT_LBM_NT_LOADS (1 << 18)
+/* prt_lbm_batch_step: the members are fluid lists (prt_lbm_segment.link) */
+#define PRT_LBM_SPARSE (1 << 1)
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *prt_last_error(void);
@@ -123,6 +125,37 @@ int prt_lbm_velocity(int q, int storage, int nx, int ny, int nz,
 /* Sum of u_x over each x-plane of v64 (3, nx, ny, nz): plane[nx]. */
 int prt_plane_flux(const double *v64, int nx, int ny, int nz, double *plane,
                    void *stream);
+
+/* ------------------------------------------------ fluid-list (sparse) LBM */
+/* The same TRT step over the fluid nodes only, a sparse pore-node list (the
+ * reference steps every box node, 3D/tools/prtlb_3d.py:403-445).  fluid_idx lists the n_fluid fluid (non-solid) box nodes in C
+ * order; populations are f[i * ld + k] for list entry k with
+ * ld = prt_lbm_pop_stride(n_fluid, storage); link[(i-1) * lk + k],
+ * lk = prt_lbm_link_stride(n_fluid), is the entry that streams into entry k
+ * along e_i (periodic) or -1 where that node is solid.  Results equal the
+ * dense PRT_LBM_SKIP_SOLID ones bit for bit. */
+int64_t prt_lbm_link_stride(int64_t n_fluid);
+/* Build the link table; map_work: n_box int32 of scratch. */
+int prt_lbm_link_table(int q, int nx, int ny, int nz, const int32_t *fluid_idx,
+                       int64_t n_fluid, int32_t *map_work, int32_t *link,
+                       void *stream);
+int prt_lbm_sparse_fill_rest(int q, int storage, int64_t n_fluid, void *f,
+                             void *stream);
+/* Partials length of the sparse run / velocity (one per 256 entries). */
+int prt_lbm_sparse_partials_count(int64_t n_fluid);
+/* As prt_lbm_run over the list (f_b must hold the state before f_a). */
+int prt_lbm_sparse_run(int q, int storage, int64_t n_fluid, const int32_t *link,
+                       void *f_a, void *f_b, int steps, double omega_p,
+                       double omega_m, double force, double fp16_scale,
+                       double *partials, void *stream);
+/* As prt_lbm_velocity (fluid-only) over the list: writes the fluid entries
+ * of v64 / v32 (3, n_box) at their box nodes; the caller zeroes the rest.
+ * partials[b]: FP64 sum of u_x per block. */
+int prt_lbm_sparse_velocity(int q, int storage, int64_t n_box, int64_t n_fluid,
+                            const int32_t *fluid_idx, const int32_t *link,
+                            const void *f_last, const void *f_prev,
+                            double force, double fp16_scale, double *v64,
+                            float *v32, double *partials, void *stream);
 
 /* --------------------------------------------------------------- transport */
 
@@ -232,6 +265,11 @@ typedef struct prt_lbm_segment {
   int32_t phase;          /* which buffer is newest at launch parity 0:
                              0 = f_a, 1 = f_b */
   double omega_p, omega_m, force, fp16_scale;
+  const int32_t *link;    /* fluid-list member (PRT_LBM_SPARSE batches): its
+                             link table (prt_lbm_link_table); f_a / f_b then
+                             hold n_fluid entries per population; NULL for a
+                             dense member.  All members of a batch agree. */
+  int64_t n_fluid;
 } prt_lbm_segment;
 
 typedef struct prt_adr_segment {

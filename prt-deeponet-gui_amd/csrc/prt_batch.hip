@@ -28,6 +28,8 @@ namespace prt {
 struct SegLbm {
   LbmArgs a;        // src/dst chosen per launch from fa/fb and the phase
   void *fa, *fb;
+  const int32_t *link;  // fluid-list member: link table (a.N = list length)
+  long long lk;
   int phase, blk0, nblk, pad;
 };
 
@@ -83,6 +85,42 @@ __global__ __launch_bounds__(256) void lbm_batch_kernel(const SegLbm *__restrict
       }
       if (RED && !solid) contrib = fabs(ux);
     }
+  }
+  if (RED) {
+    const double t = block_sum(contrib, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+  }
+}
+
+// Fluid-list members: entry k of member s's list, the single-geometry
+// sparse step's code (gather_sparse, the same collision and scatter).
+template <int Q, int ST, bool RED>
+__global__ __launch_bounds__(256) void lbm_batch_sparse_kernel(const SegLbm *__restrict__ segs,
+                                                               const int *__restrict__ blk_seg, int parity,
+                                                               double *partials) {
+  __shared__ double lds[16];
+  const int s = __builtin_amdgcn_readfirstlane(blk_seg[blockIdx.x]);
+  const SegLbm &d = segs[s];
+  LbmArgs a = d.a;
+  const bool newest_a = ((parity ^ d.phase) & 1) == 0;
+  a.src = newest_a ? d.fa : d.fb;
+  a.dst = newest_a ? d.fb : d.fa;
+  const int k = (int)(blockIdx.x - d.blk0) * 256 + threadIdx.x;
+  double contrib = 0.0;
+  if (k < a.N) {
+    double ux;
+    if constexpr (ST == PRT_STORE_FP64) {
+      double f[Q], fp[Q];
+      gather_sparse<Q, ST>(a, d.link, d.lk, k, f);
+      collide_ref<Q>(a, false, f, fp, ux);
+      scatter<Q, ST>(a, k, fp);
+    } else {
+      float f[Q], fp[Q];
+      gather_sparse<Q, ST>(a, d.link, d.lk, k, f);
+      collide_dev<Q>(a, false, f, fp, ux);
+      scatter<Q, ST>(a, k, fp);
+    }
+    if (RED) contrib = fabs(ux);
   }
   if (RED) {
     const double t = block_sum(contrib, lds);
@@ -156,8 +194,10 @@ extern "C" int prt_lbm_batch_plan(int q, int storage, const prt_lbm_segment *seg
   long long total = 0;
   for (int s = 0; s < nseg; ++s) {
     const prt_lbm_segment &g = segs[s];
-    if (g.nx < 1 || g.ny < 1 || g.nz < 1 || (q == 9 && g.nz != 1) || !g.solid || !g.f_a || !g.f_b ||
-        g.f_a == g.f_b) {
+    const bool sparse = g.link != nullptr;
+    if (g.nx < 1 || g.ny < 1 || g.nz < 1 || (q == 9 && g.nz != 1) || (!sparse && !g.solid) || !g.f_a ||
+        !g.f_b || g.f_a == g.f_b || (sparse && (g.n_fluid < 1 || g.n_fluid > (long long)g.nx * g.ny * g.nz)) ||
+        (s > 0 && sparse != (segs[0].link != nullptr))) {
       set_error("prt_lbm_batch_plan: segment %d has a bad grid or buffers", s);
       return PRT_E_ARG;
     }
@@ -174,7 +214,7 @@ extern "C" int prt_lbm_batch_plan(int q, int storage, const prt_lbm_segment *seg
     } else {
       a.X = g.nx; a.Y = g.ny; a.Z = g.nz;
     }
-    a.N = (long long)g.nx * g.ny * g.nz;
+    a.N = sparse ? (long long)g.n_fluid : (long long)g.nx * g.ny * g.nz;
     a.ld = pop_stride(a.N, storage == PRT_STORE_FP64 ? 8 : (storage == PRT_STORE_FP32 ? 4 : 2));
     if (a.ld * q >= (1LL << 31)) {
       set_error("prt_lbm_batch_plan: segment %d too large for the 32-bit index", s);
@@ -187,6 +227,8 @@ extern "C" int prt_lbm_batch_plan(int q, int storage, const prt_lbm_segment *seg
     a.inv_s16 = (float)(1.0 / s16);
     d[s].fa = g.f_a;
     d[s].fb = g.f_b;
+    d[s].link = g.link;
+    d[s].lk = sparse ? link_stride(a.N) : 0;
     d[s].phase = g.phase & 1;
     d[s].blk0 = (int)total;
     d[s].nblk = blocks_of(a.N);
@@ -212,9 +254,13 @@ extern "C" int prt_lbm_batch_step(int q, int storage, const void *desc_dev, cons
   const SegLbm *d = static_cast<const SegLbm *>(desc_dev);
   hipStream_t st = (hipStream_t)stream;
   const bool skip = (flags & PRT_LBM_SKIP_SOLID) != 0, red = partials != nullptr;
+  const bool sparse = (flags & PRT_LBM_SPARSE) != 0;
   const dim3 grid((unsigned)n_blocks), block(256);
 #define PRT_LB(QQ, SS)                                                                       \
   do {                                                                                       \
+    if (sparse && red) lbm_batch_sparse_kernel<QQ, SS, true><<<grid, block, 0, st>>>(d, blk_seg_dev, parity, partials); \
+    else if (sparse) lbm_batch_sparse_kernel<QQ, SS, false><<<grid, block, 0, st>>>(d, blk_seg_dev, parity, partials); \
+    else                                                                                     \
     if (skip && red) lbm_batch_kernel<QQ, SS, true, true><<<grid, block, 0, st>>>(d, blk_seg_dev, parity, partials);  \
     else if (skip) lbm_batch_kernel<QQ, SS, true, false><<<grid, block, 0, st>>>(d, blk_seg_dev, parity, partials);   \
     else if (red) lbm_batch_kernel<QQ, SS, false, true><<<grid, block, 0, st>>>(d, blk_seg_dev, parity, partials);    \

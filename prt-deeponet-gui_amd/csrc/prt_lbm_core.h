@@ -62,6 +62,9 @@ struct LbmArgs {
 // for every entry point, so buffers sized with prt_lbm_pop_stride() fit.
 long long pop_stride(long long n, int elem_bytes);
 
+// Element stride of the link-table rows of an n_fluid fluid list.
+long long link_stride(long long n_fluid);
+
 // Guo forcing term w_i (1 - omm/2) 3 g e_ix with the reference's association
 // (3D/tools/prtlb_3d.py:439-440); exact constants, contraction off.
 template <int Q>
@@ -139,6 +142,33 @@ __device__ inline void gather(const LbmArgs &a, const Nbr &nb, int n, R *f) {
       v = __builtin_nontemporal_load(src + (size_t)i * a.ld + s);   // read once: do not keep
     else
       v = src[(size_t)i * a.ld + s];
+    if constexpr (ST == PRT_STORE_FP16)
+      f[i] = __half2float(v) * ld16<Q>(a, i);
+    else
+      f[i] = (R)v;
+  }
+}
+
+// Sparse (fluid-list) storage: only the fluid nodes' populations,
+// f[i * ld + k] for list entry k; link[(i - 1) * lk + k] is the list entry
+// of the node that streams into entry k along e_i, or -1 where that node is
+// solid -- then, as in gather<SKIP>, the entry's own two-step-old opposite
+// slot (still in the write buffer).  Same values as the dense fluid-only
+// gather, so the same bits after the collision.
+template <int Q, int ST, typename R>
+__device__ inline void gather_sparse(const LbmArgs &a, const int32_t *__restrict__ link, long long lk, int k,
+                                     R *f) {
+  using T = typename Pop<ST>::T;
+  const T *src = static_cast<const T *>(a.src);
+  const T *own = static_cast<const T *>(a.dst);
+  int s[Q];
+  s[0] = k;
+#pragma unroll
+  for (int i = 1; i < Q; ++i) s[i] = link[(size_t)(i - 1) * lk + k];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const T *p = (i == 0 || s[i] >= 0) ? src + (size_t)i * a.ld + s[i] : own + (size_t)Lat<Q>::opp[i] * a.ld + k;
+    const T v = *p;
     if constexpr (ST == PRT_STORE_FP16)
       f[i] = __half2float(v) * ld16<Q>(a, i);
     else

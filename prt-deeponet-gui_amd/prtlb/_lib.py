@@ -18,6 +18,7 @@ PRT_LBM_SKIP_SOLID = 1
 
 
 PRT_LBM_NT_LOADS = 1 << 18
+PRT_LBM_SPARSE = 1 << 1
 
 
 def lbm_tune(bz_log2, by_log2, npt, nt_loads=False):
@@ -56,7 +57,8 @@ class LbmSegment(ctypes.Structure):
                 ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
                 ("phase", ctypes.c_int32), ("omega_p", ctypes.c_double),
                 ("omega_m", ctypes.c_double), ("force", ctypes.c_double),
-                ("fp16_scale", ctypes.c_double)]
+                ("fp16_scale", ctypes.c_double), ("link", ctypes.c_void_p),
+                ("n_fluid", ctypes.c_int64)]
 
 
 class AdrSegment(ctypes.Structure):
@@ -86,6 +88,12 @@ _SIGS = {
     "prt_adr_step": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _P]),
     "prt_adr_step_f64": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _P]),
     "prt_adr_run": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _I, _P]),
+    "prt_lbm_link_stride": (_I64, [_I64]),
+    "prt_lbm_link_table": (_I, [_I, _I, _I, _I, _P, _I64, _P, _P, _P]),
+    "prt_lbm_sparse_fill_rest": (_I, [_I, _I, _I64, _P, _P]),
+    "prt_lbm_sparse_partials_count": (_I, [_I64]),
+    "prt_lbm_sparse_run": (_I, [_I, _I, _I64, _P, _P, _P, _I, _D, _D, _D, _D, _P, _P]),
+    "prt_lbm_sparse_velocity": (_I, [_I, _I, _I64, _I64, _P, _P, _P, _P, _D, _D, _P, _P, _P, _P]),
     "prt_geodesic": (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _I, _P, _P, _P]),
     "prt_edt": (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P]),
     "prt_check_blocks": (_I, [_I64]),

@@ -18,8 +18,8 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import (PRT_LBM_SKIP_SOLID, PRT_STORE, AdrSegment, LbmSegment,
-                   PrtError, call, lib)
+from ._lib import (PRT_LBM_SKIP_SOLID, PRT_LBM_SPARSE, PRT_STORE, AdrSegment,
+                   LbmSegment, PrtError, call, lib)
 from .adr import AdrRun, TransportSolver
 from .device import device as _device
 from .device import ptr, stream_ptr
@@ -29,11 +29,15 @@ from .stokes import DTYPES, FlowConvergence, checked_flow, trt_minus
 _STORE_BYTES = {"fp64": 8, "fp32": 4, "fp16": 2}
 
 
-def lbm_footprint(shape, storage):
-    """Device bytes one Stokes member holds (two population sets + mask)."""
+def lbm_footprint(shape, storage, n_fluid=None):
+    """Device bytes one Stokes member holds: two population sets + mask over
+    the box, or, for a fluid list of n_fluid nodes, two compact population
+    sets, the link table, the list and the mask."""
     n = int(np.prod(shape))
     q = 9 if len(shape) == 2 else 19
-    return n * (2 * q * _STORE_BYTES[storage] + 1)
+    if n_fluid is None:
+        return n * (2 * q * _STORE_BYTES[storage] + 1)
+    return int(n_fluid) * (2 * q * _STORE_BYTES[storage] + 4 * (q - 1) + 4) + n
 
 
 def adr_footprint(shape, ns, nd):
@@ -59,11 +63,16 @@ class LbmBatch:
     """
 
     def __init__(self, q, storage="fp32", skip_solid=True, every=50, device=None,
-                 fp16_scale=1024.0):
+                 fp16_scale=1024.0, sparse=None):
         if q not in (9, 19):
             raise ValueError("q must be 9 or 19")
         self.q, self.storage, self.st = q, storage, PRT_STORE[storage]
-        self.flags = PRT_LBM_SKIP_SOLID if skip_solid else 0
+        # fluid-list members (compact populations + link table) by default
+        # in the fluid-only mode; the dense box otherwise
+        self.sparse = bool(skip_solid) if sparse is None else bool(sparse)
+        if self.sparse and not skip_solid:
+            raise ValueError("a fluid-list batch is fluid-only (skip_solid=True)")
+        self.flags = (PRT_LBM_SKIP_SOLID if skip_solid else 0) | (PRT_LBM_SPARSE if self.sparse else 0)
         self.every = int(every)
         self.dev = _device(device)
         self.fp16_scale = float(fp16_scale)
@@ -96,18 +105,36 @@ class LbmBatch:
         m.force, m.tau = float(force), float(tau_lb)
         m.om_p = 1.0 / m.tau
         m.om_m = trt_minus(m.om_p, magic)
-        ld = int(lib().prt_lbm_pop_stride(m.n, self.st))
-        m.f = [torch.empty((self.q, ld), dtype=DTYPES[self.storage], device=self.dev)
-               for _ in range(2)]
         s = stream_ptr(self.dev)
-        for buf in m.f:
-            call("prt_lbm_fill_rest", self.q, self.st, m.n, ptr(buf), s)
+        if self.sparse:
+            if m.n_fluid < 1:
+                raise PrtError("a fluid-list member needs at least one fluid node")
+            m.fidx = torch.from_numpy(np.flatnonzero(g.reshape(-1) == PORE).astype(np.int32)).to(self.dev)
+            lk = int(lib().prt_lbm_link_stride(m.n_fluid))
+            m.link = torch.empty((self.q - 1) * lk, dtype=torch.int32, device=self.dev)
+            work = torch.empty(m.n, dtype=torch.int32, device=self.dev)
+            call("prt_lbm_link_table", self.q, *m.dims, ptr(m.fidx), m.n_fluid, ptr(work),
+                 ptr(m.link), s)
+            m.nodes = m.n_fluid
+            ld = int(lib().prt_lbm_pop_stride(m.n_fluid, self.st))
+            m.f = [torch.empty((self.q, ld), dtype=DTYPES[self.storage], device=self.dev)
+                   for _ in range(2)]
+            for buf in m.f:
+                call("prt_lbm_sparse_fill_rest", self.q, self.st, m.n_fluid, ptr(buf), s)
+            torch.cuda.current_stream(self.dev).synchronize()   # work is freed on return
+        else:
+            m.link, m.nodes = None, m.n
+            ld = int(lib().prt_lbm_pop_stride(m.n, self.st))
+            m.f = [torch.empty((self.q, ld), dtype=DTYPES[self.storage], device=self.dev)
+                   for _ in range(2)]
+            for buf in m.f:
+                call("prt_lbm_fill_rest", self.q, self.st, m.n, ptr(buf), s)
         m.phase = self.parity & 1            # newest = f[0] at the current parity
         m.iters = 0
         m.nit = int(nit)
         m.conv = FlowConvergence(tol=tol, every=self.every, max_it=nit,
                                  name="%dD flow" % nd)
-        m.bytes = lbm_footprint(g.shape, self.storage)
+        m.bytes = lbm_footprint(g.shape, self.storage, m.n_fluid if self.sparse else None)
         self.members[key] = m
         self._dirty = True
         return m
@@ -124,7 +151,8 @@ class LbmBatch:
         segs = (LbmSegment * len(mem))()
         for i, m in enumerate(mem):
             segs[i] = LbmSegment(ptr(m.solid), ptr(m.f[0]), ptr(m.f[1]), *m.dims, m.phase,
-                                 m.om_p, m.om_m, m.force, self.fp16_scale)
+                                 m.om_p, m.om_m, m.force, self.fp16_scale,
+                                 ptr(m.link) if m.link is not None else None, m.nodes)
         dsz = lib().prt_lbm_batch_desc_size()
         desc = np.zeros(dsz * len(mem), np.uint8)
         nb = ctypes.c_int64()
@@ -140,9 +168,9 @@ class LbmBatch:
         blk0 = np.zeros(len(mem), np.int32)
         nblk = np.zeros(len(mem), np.int32)
         for i in range(1, len(mem)):
-            blk0[i] = blk0[i - 1] + (mem[i - 1].n + 255) // 256
+            blk0[i] = blk0[i - 1] + (mem[i - 1].nodes + 255) // 256
         for i, m in enumerate(mem):
-            nblk[i] = (m.n + 255) // 256
+            nblk[i] = (m.nodes + 255) // 256
         self.blk0 = torch.from_numpy(blk0).to(self.dev)
         self.nblk = torch.from_numpy(nblk).to(self.dev)
         self.partials = torch.zeros(self.n_blocks, dtype=torch.float64, device=self.dev)
@@ -183,15 +211,23 @@ class LbmBatch:
         """Velocity of a finished member (float32 (3, *grid) device tensor,
         zero in solid) and its info; removes it from the batch."""
         m.conv.report(info)
-        v32 = torch.empty((3, m.n), dtype=torch.float32, device=self.dev)
-        part = torch.zeros(lib().prt_lbm_partials_count(self.q, *m.dims), dtype=torch.float64,
-                           device=self.dev)
-        out = torch.zeros(1, dtype=torch.float64, device=self.dev)
         s = stream_ptr(self.dev)
         newest = self.newest(m)
         other = m.f[1] if newest is m.f[0] else m.f[0]
-        call("prt_lbm_velocity", self.q, self.st, *m.dims, ptr(m.solid), ptr(newest), ptr(other),
-             self.flags, m.force, self.fp16_scale, None, ptr(v32), ptr(part), s)
+        out = torch.zeros(1, dtype=torch.float64, device=self.dev)
+        if self.sparse:
+            v32 = torch.zeros((3, m.n), dtype=torch.float32, device=self.dev)
+            part = torch.zeros(lib().prt_lbm_sparse_partials_count(m.n_fluid), dtype=torch.float64,
+                               device=self.dev)
+            call("prt_lbm_sparse_velocity", self.q, self.st, m.n, m.n_fluid, ptr(m.fidx),
+                 ptr(m.link), ptr(newest), ptr(other), m.force, self.fp16_scale, None, ptr(v32),
+                 ptr(part), s)
+        else:
+            v32 = torch.empty((3, m.n), dtype=torch.float32, device=self.dev)
+            part = torch.zeros(lib().prt_lbm_partials_count(self.q, *m.dims), dtype=torch.float64,
+                               device=self.dev)
+            call("prt_lbm_velocity", self.q, self.st, *m.dims, ptr(m.solid), ptr(newest),
+                 ptr(other), self.flags, m.force, self.fp16_scale, None, ptr(v32), ptr(part), s)
         call("prt_sum_f64", ptr(part), part.numel(), ptr(out), s)
         sux = float(out.item())
         v32 = v32.reshape((3,) + m.shape)

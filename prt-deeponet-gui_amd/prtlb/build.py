@@ -21,6 +21,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH,
 SOURCES = {
     "prt_common.hip": [],
     "prt_lbm.hip": [],
+    "prt_sparse.hip": [],
     "prt_adr.hip": ["-ffp-contract=off"],
     "prt_reduce.hip": ["-ffp-contract=off"],
     "prt_batch.hip": [],

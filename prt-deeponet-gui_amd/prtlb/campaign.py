@@ -178,7 +178,8 @@ def run_batched(specs, geoms, storage="fp32", tau=0.8, tol=1e-6, nit=30000,
         while pending:
             key, stage = pending[0]
             g = geoms[key]
-            if len(lb) and lb.bytes() + lbm_footprint(g.shape, storage) > hbm_budget:
+            need = lbm_footprint(g.shape, storage, int((g == PORE).sum()) if lb.sparse else None)
+            if len(lb) and lb.bytes() + need > hbm_budget:
                 break
             pending.pop(0)
             force = PROBE_FORCE if stage == "probe" else res[key]["force"]

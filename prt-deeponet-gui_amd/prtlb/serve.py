@@ -23,6 +23,7 @@ import torch
 
 from . import campaign as C
 from .batch import AdrBatch, LbmBatch, adr_footprint, lbm_footprint
+from .geometry import PORE
 
 
 class Request:
@@ -54,7 +55,9 @@ class Server:
     def _footprint(self, req):
         g = req.g
         kw = C._adr_kw(dict(da=req.da), req.kinetics)
-        return max(lbm_footprint(g.shape, self.storage), adr_footprint(g.shape, kw["n_species"], g.ndim))
+        n_fluid = int((g == PORE).sum()) if self.lb[9].sparse else None
+        return max(lbm_footprint(g.shape, self.storage, n_fluid),
+                   adr_footprint(g.shape, kw["n_species"], g.ndim))
 
     def in_use(self):
         return sum(self.reserved.values())

@@ -12,10 +12,11 @@ from prtlb.serve import Request, Server
 pytestmark = pytest.mark.gpu
 
 
-def test_lbm_batch_equals_single():
+@pytest.mark.parametrize("sparse", [True, False])
+def test_lbm_batch_equals_single(sparse):
     geoms = [prtlb.sphere_pack((20, 16, 12), (2.0, 3.5), 0.55, seed=s) for s in range(3)]
     geoms.append(prtlb.sphere_pack((33, 17, 9), (2.0, 3.0), 0.6, seed=9))
-    lb = LbmBatch(19, storage="fp32")
+    lb = LbmBatch(19, storage="fp32", sparse=sparse)
     for i, g in enumerate(geoms):
         lb.add(i, g, force=1e-5 * (i + 1), nit=300, tol=0.0)
     for _ in range(6):
@@ -46,6 +47,77 @@ def test_lbm_batch_join_late_and_converge():
         v = prtlb.stokes_d2q9(g, nit=20000, force=2e-6, tol=1e-6, info=info)
         assert np.array_equal(got[key][0], v)
         assert got[key][1]["flow_iters"] == info["flow_iters"]
+
+
+@pytest.mark.parametrize("q,storage", [(19, "fp64"), (19, "fp32"), (19, "fp16"), (9, "fp64"),
+                                       (9, "fp32")])
+def test_fluid_list_step_equals_dense_fluid_only(q, storage):
+    """The fluid-list kernels against the dense fluid-only kernels: the
+    compact populations equal the dense ones at every fluid node, and the
+    velocity fields are equal, bit for bit."""
+    import ctypes
+    import torch
+    from prtlb import _lib
+    from prtlb.device import ptr, stream_ptr
+    from prtlb.stokes import StokesSolver
+    g = (prtlb.sphere_pack((26, 18, 14), (2.0, 4.0), 0.5, seed=11) if q == 19
+         else prtlb.disk_pack((50, 34), 3.0, 0.6, seed=12))
+    dims = (g.shape[0], g.shape[1], 1 if g.ndim == 2 else g.shape[2])
+    force, steps = 3e-5, 137
+    S = StokesSolver(g, force=force, storage=storage, skip_solid=True)
+    S.run(steps)
+    dev = S.dev
+    st = _lib.PRT_STORE[storage]
+    fidx = torch.from_numpy(np.flatnonzero(g.reshape(-1) == prtlb.PORE).astype(np.int32)).to(dev)
+    nf = fidx.numel()
+    lk = _lib.lib().prt_lbm_link_stride(nf)
+    link = torch.empty((q - 1) * lk, dtype=torch.int32, device=dev)
+    work = torch.empty(g.size, dtype=torch.int32, device=dev)
+    s = stream_ptr(dev)
+    _lib.call("prt_lbm_link_table", q, *dims, ptr(fidx), nf, ptr(work), ptr(link), s)
+    ld = _lib.lib().prt_lbm_pop_stride(nf, st)
+    fa = torch.empty((q, ld), dtype=S.f[0].dtype, device=dev)
+    fb = torch.empty_like(fa)
+    for b in (fa, fb):
+        _lib.call("prt_lbm_sparse_fill_rest", q, st, nf, ptr(b), s)
+    _lib.call("prt_lbm_sparse_run", q, st, nf, ptr(link), ptr(fa), ptr(fb), steps, S.om_p, S.om_m,
+              force, S.fp16_scale, None, s)
+    newest, other = (fb, fa) if steps % 2 else (fa, fb)
+    dense_new = S.f[S.latest] if hasattr(S, "latest") else None
+    v_dense = S.velocity(f64=True)[1].cpu().numpy()
+    v64 = torch.zeros((3, g.size), dtype=torch.float64, device=dev)
+    part = torch.zeros(_lib.lib().prt_lbm_sparse_partials_count(nf), dtype=torch.float64, device=dev)
+    _lib.call("prt_lbm_sparse_velocity", q, st, g.size, nf, ptr(fidx), ptr(link), ptr(newest),
+              ptr(other), force, S.fp16_scale, ptr(v64), None, ptr(part), s)
+    assert np.array_equal(v64.cpu().numpy().reshape(v_dense.shape), v_dense)
+    if dense_new is not None:
+        d = np.ascontiguousarray(dense_new[:, :g.size].cpu().numpy()[:, fidx.cpu().numpy()])
+        c = np.ascontiguousarray(newest[:, :nf].cpu().numpy())
+        assert np.array_equal(d.view(np.uint8), c.view(np.uint8))
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp32"])
+def test_fluid_list_batch_join_late_and_converge(storage):
+    g1 = prtlb.sphere_pack((24, 18, 14), (2.0, 4.0), 0.5, seed=21)
+    g2 = prtlb.sphere_pack((30, 12, 16), (2.0, 3.0), 0.55, seed=22)
+    lb = LbmBatch(19, storage=storage)
+    assert lb.sparse
+    lb.add("a", g1, force=2e-6, tol=1e-6, nit=20000)
+    for _ in range(3):
+        lb.tick()
+    lb.add("b", g2, force=2e-6, tol=1e-6, nit=20000)
+    got = {}
+    while len(lb):
+        for m in lb.tick():
+            info = {}
+            got[m.key] = (lb.finish(m, info).cpu().numpy(), info)
+    for key, g in (("a", g1), ("b", g2)):
+        info = {}
+        v = prtlb.stokes_d3q19(g, nit=20000, force=2e-6, tol=1e-6, info=info, storage=storage,
+                               skip_solid=True)
+        assert np.array_equal(got[key][0], v)
+        assert got[key][1]["flow_iters"] == info["flow_iters"]
+        assert got[key][1]["permeability"] == pytest.approx(info["permeability"], rel=1e-12)
 
 
 def test_adr_batch_equals_single():
