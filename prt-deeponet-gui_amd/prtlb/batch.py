@@ -147,6 +147,8 @@ class LbmBatch:
         return m.f[0] if ((self.parity ^ m.phase) & 1) == 0 else m.f[1]
 
     def _plan(self):
+        # captured graphs hold the previous plan's pointers: drop them
+        self._graphs = {}
         mem = list(self.members.values())
         segs = (LbmSegment * len(mem))()
         for i, m in enumerate(mem):
@@ -177,28 +179,57 @@ class LbmBatch:
         self.sums = torch.zeros(len(mem), dtype=torch.float64, device=self.dev)
         self._dirty = False
 
+    def _launch(self, parity, steps, reduce_last):
+        s = stream_ptr(self.dev)
+        for k in range(steps):
+            red = reduce_last and k == steps - 1
+            call("prt_lbm_batch_step", self.q, self.st, ptr(self.desc), ptr(self.blk_seg),
+                 self.n_blocks, (parity + k) & 1, self.flags,
+                 ptr(self.partials) if red else None, s)
+
     def step(self, steps, reduce_last=False):
         """Advance every member `steps` iterations (one launch each)."""
         if not self.members or steps <= 0:
             return
         if self._dirty:
             self._plan()
-        s = stream_ptr(self.dev)
-        for k in range(steps):
-            red = reduce_last and k == steps - 1
-            call("prt_lbm_batch_step", self.q, self.st, ptr(self.desc), ptr(self.blk_seg),
-                 self.n_blocks, self.parity & 1, self.flags,
-                 ptr(self.partials) if red else None, s)
-            self.parity += 1
+        self._launch(self.parity, steps, reduce_last)
+        self.parity += steps
         for m in self.members.values():
             m.iters += steps
+
+    # ticks replay a captured HIP graph of the `every` launches and the
+    # per-member sums while the membership (the plan) is unchanged; every
+    # is even, so each tick starts at the same launch parity
+    use_graph = True
+
+    def _tick_launches(self):
+        if self._dirty:
+            self._plan()
+        if not (self.use_graph and self.every % 2 == 0):
+            self._launch(self.parity, self.every, True)
+            call("prt_seg_sums", ptr(self.blk0), ptr(self.nblk), len(self.order), 1,
+                 ptr(self.partials), ptr(self.sums), stream_ptr(self.dev))
+            return
+        key = self.parity & 1
+        if key not in self._graphs:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._launch(self.parity, self.every, True)
+                call("prt_seg_sums", ptr(self.blk0), ptr(self.nblk), len(self.order), 1,
+                     ptr(self.partials), ptr(self.sums), stream_ptr(self.dev))
+            self._graphs[key] = g
+        self._graphs[key].replay()
 
     def tick(self):
         """`every` iterations, then the convergence check of every member.
         Returns the members that finished (converged or hit their cap)."""
-        self.step(self.every, reduce_last=True)
-        call("prt_seg_sums", ptr(self.blk0), ptr(self.nblk), len(self.order), 1,
-             ptr(self.partials), ptr(self.sums), stream_ptr(self.dev))
+        if not self.members:
+            return []
+        self._tick_launches()
+        self.parity += self.every
+        for m in self.members.values():
+            m.iters += self.every
         sums = self.sums.cpu().numpy()
         done = []
         for i, m in enumerate(self.order):
@@ -281,7 +312,31 @@ class AdrBatch:
         for m in self.members.values():
             m.T.latest = 0 if ((self.parity ^ m.phase) & 1) == 0 else 1
 
+    # chunks of GRAPH_STEPS (even) launches replay a captured HIP graph while
+    # the membership (the plan) is unchanged
+    GRAPH_STEPS = 32
+    use_graph = True
+
+    def _launch(self, parity, steps):
+        s = stream_ptr(self.dev)
+        for k in range(steps):
+            call("prt_adr_batch_step", ptr(self.desc), ptr(self.blk_seg), self.n_blocks,
+                 self.max_species, (parity + k) & 1, None, s)
+
+    def _graph_chunk(self):
+        graphs = getattr(self, "_graphs", None)
+        if graphs is None or graphs[0] != self._plan_id:
+            graphs = self._graphs = (self._plan_id, {})
+        key = self.parity & 1
+        if key not in graphs[1]:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._launch(self.parity, self.GRAPH_STEPS)
+            graphs[1][key] = g
+        graphs[1][key].replay()
+
     def _plan(self):
+        self._plan_id = getattr(self, "_plan_id", 0) + 1
         mem = list(self.members.values())
         segs = (AdrSegment * len(mem))()
         for i, m in enumerate(mem):
@@ -311,11 +366,13 @@ class AdrBatch:
         if self._dirty:
             self._plan()
         todo = min(m.run.next_target() - m.run.n for m in self.members.values())
-        s = stream_ptr(self.dev)
-        for _ in range(todo):
-            call("prt_adr_batch_step", ptr(self.desc), ptr(self.blk_seg), self.n_blocks,
-                 self.max_species, self.parity & 1, None, s)
-            self.parity += 1
+        left = todo
+        while self.use_graph and left >= self.GRAPH_STEPS:
+            self._graph_chunk()
+            self.parity += self.GRAPH_STEPS
+            left -= self.GRAPH_STEPS
+        self._launch(self.parity, left)
+        self.parity += left
         self._sync_latest()
         done = []
         for m in list(self.members.values()):
