@@ -61,6 +61,32 @@ def time_lbm(g, storage, steps, warmup, skip_solid=False, tune=True):
     return out
 
 
+def time_fluid_list(g, storage, steps, warmup):
+    """One geometry as a fluid-list member (compact populations + link
+    table): time per step, fluid-node updates per second, and bandwidth
+    counting the populations (2 Q s per fluid node) and the link table
+    (4 (Q - 1) bytes) it reads."""
+    import torch
+    from prtlb.batch import LbmBatch
+    b = LbmBatch(19, storage=storage)
+    b.add(0, g, force=1e-6)
+    b.step(warmup)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    b.step(steps)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    nf = b.members[0].n_fluid
+    sb = {"fp64": 8, "fp32": 4, "fp16": 2}[storage]
+    pop_b, link_b = 2 * 19 * sb, 4 * 18
+    gbs = nf * (pop_b + link_b) / (ms * 1e-3) / 1e9
+    return dict(ms_per_step=ms, mlups_fluid=nf / (ms * 1e-3) / 1e6, fluid_nodes=nf,
+                bytes_per_fluid_node=pop_b + link_b, gbs=gbs, frac=gbs * 1e9 / PEAK_HBM)
+
+
 def time_adr(g, steps, warmup, kinetics="monod", pore_list=True):
     import numpy as np
     import torch
@@ -329,6 +355,10 @@ def main():
         log("fluid-only fp64 128^3 ...")
         extras["d3q19_128_fp64_fluid_only"] = time_lbm(g128, "fp64", args.steps, args.warmup,
                                                        skip_solid=True)
+        for st in ("fp64", "fp32"):
+            log("fluid list", st, "128^3 ...")
+            extras["d3q19_128_%s_fluid_list" % st] = time_fluid_list(g128, st, args.steps,
+                                                                     args.warmup)
         log("geometry 256^3 ...")
         g256 = prtlb.sphere_pack((256, 256, 256), (4.0, 8.0), 0.35, seed=0)
         for st in ("fp64", "fp32", "fp16"):

@@ -172,3 +172,36 @@ def test_campaign_batched_matches_sequential_fp32():
         assert rb[k]["flow_iters"] == rs[k]["flow_iters"]
         assert rb[k]["permeability"] == pytest.approx(rs[k]["permeability"], rel=1e-12)
         assert np.array_equal(rb[k]["out_last"], rs[k]["out_last"])
+
+
+def test_dataset_layout_and_content():
+    """The campaign dataset: the reference's group/dataset names, dtypes and
+    shapes; each stored run equals the same transport solve alone."""
+    from prtlb import dataset as D
+    ds = D.build_dataset(n_geom=2, shape=(24, 20, 20), n_sets=2, n_times=4, radius=(2.0, 4.0),
+                         adr_steps=300, seed=3)
+    G, S = 2, 4
+    assert ds["geom/material"].shape == (G, 24, 20, 20) and ds["geom/material"].dtype == np.uint8
+    assert ds["geom/gdf"].dtype == np.float32 and ds["geom/edt"].dtype == np.float32
+    assert ds["samples/conc"].shape == (S, 4, 2, 24, 20, 20)
+    assert ds["samples/conc"].dtype == np.float16 and ds["samples/velocity"].dtype == np.float16
+    assert ds["samples/conc.attrs.conc_scale"].shape == (2,)
+    assert list(ds["samples/geom_index"]) == [0, 0, 1, 1]
+    from oracle import prtlb_oracle as O
+    for i in range(G):
+        ref, _ = O.geodesic(ds["geom/material"][i])
+        assert np.array_equal(ds["geom/gdf"][i], ref)
+    k = 3
+    i, pe, da = ds["_draws"][k]
+    g = ds["geom/material"][i]
+    v = ds["_flows"][i]
+    assert np.array_equal(ds["samples/velocity"][k], v.astype(np.float16))
+    kw = C._adr_kw(dict(da=da), "sorption_irreversible")
+    out, tn = prtlb.solve_adr(g, v, pe, kw.pop("da"), 4, steps=300, **kw)
+    assert np.array_equal(ds["samples/conc"][k], out.astype(np.float16))
+    assert np.array_equal(ds["samples/t_norm"][k], tn)
+    import os, tempfile
+    path = os.path.join(tempfile.mkdtemp(), "ds.npz")
+    D.write_dataset(path, ds)
+    z = np.load(path, allow_pickle=False)
+    assert "samples/conc" in z.files and z["samples/settled"].dtype == bool
