@@ -39,9 +39,12 @@ template <typename R>
 __device__ inline R npmax(R a, R b) {
   return (a >= b || a != a) ? a : b;
 }
+// np.sign: +-1 with the sign of d, and d itself for +-0 and NaN (one class
+// test and a sign copy instead of two compares).
 template <typename R>
 __device__ inline R npsign(R d) {
-  return d > R(0) ? R(1) : (d < R(0) ? R(-1) : d);
+  const bool keep = __builtin_amdgcn_class(d, 0x63);  // NaN, -0 or +0
+  return keep ? d : __builtin_copysign(R(1), d);
 }
 
 // C_face = up + 1/2 minmod(down - up, up - upup), 3D/tools/prtlb_3d.py:700-705

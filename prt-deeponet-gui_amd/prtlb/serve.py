@@ -39,7 +39,13 @@ class Server:
     """Continuous-batching PRT-LB server on one GPU."""
 
     def __init__(self, hbm_budget=200e9, storage="fp32", tau=0.8, tol=1e-6, nit=30000, n_t=5,
-                 adr_steps=None, max_steps=100000, device=None):
+                 adr_steps=None, max_steps=100000, device=None, transport_cap=24e6):
+        # transport_cap: pore voxels in the transport batch at once.  Enough
+        # to keep the device busy; requests beyond it wait, first come first
+        # served, so early requests finish early instead of every request
+        # sharing the device until the end (None: no cap).
+        self.transport_cap = transport_cap
+        self.ready = deque()
         self.budget = float(hbm_budget)
         self.storage, self.tau, self.tol, self.nit = storage, tau, tol, nit
         self.n_t, self.adr_steps, self.max_steps = n_t, adr_steps, max_steps
@@ -95,10 +101,8 @@ class Server:
                     continue
                 req.permeability = info["permeability"]
                 req.velocity = v
-                kw = C._adr_kw(dict(da=req.da), req.kinetics)
-                am = self.ab.add(key, req.g, v, req.pe, kw.pop("da"), self.n_t,
-                                 steps=self.adr_steps, max_steps=self.max_steps, **kw)
-                am.req = req
+                self.ready.append(req)
+        self._admit_transport()
         if len(self.ab):
             for m in self.ab.advance():
                 req = m.req
@@ -110,8 +114,22 @@ class Server:
                 self.reserved.pop(req.key, None)
                 self.done.append(req)
 
+    def _admit_transport(self):
+        while self.ready:
+            req = self.ready[0]
+            n_pore = int((req.g == PORE).sum())
+            held = sum(m.T.n_pore for m in self.ab.members.values())
+            if self.transport_cap and len(self.ab) and held + n_pore > self.transport_cap:
+                break
+            self.ready.popleft()
+            kw = C._adr_kw(dict(da=req.da), req.kinetics)
+            am = self.ab.add(req.key, req.g, req.velocity, req.pe, kw.pop("da"), self.n_t,
+                             steps=self.adr_steps, max_steps=self.max_steps, **kw)
+            am.req = req
+
     def idle(self):
-        return not (self.waiting or any(len(b) for b in self.lb.values()) or len(self.ab))
+        return not (self.waiting or self.ready or any(len(b) for b in self.lb.values())
+                    or len(self.ab))
 
     def run(self, requests, realtime=True, progress=None):
         """Serve `requests` (arrival times in seconds from now when realtime,
@@ -125,11 +143,12 @@ class Server:
         while not self.idle():
             if progress and time.perf_counter() - t_note > 20.0:
                 t_note = time.perf_counter()
-                progress("serve %.0f s: %d done, %d in flow, %d in transport, %d waiting, %.1f GB held"
+                progress("serve %.0f s: %d done, %d in flow, %d ready, %d in transport, %d waiting, "
+                         "%.1f GB held"
                          % (t_note - t0, len(self.done), sum(len(b) for b in self.lb.values()),
-                            len(self.ab), len(self.waiting), self.in_use() / 1e9))
+                            len(self.ready), len(self.ab), len(self.waiting), self.in_use() / 1e9))
             now = (time.perf_counter() - t0) if realtime else float("inf")
-            busy = any(len(b) for b in self.lb.values()) or len(self.ab)
+            busy = any(len(b) for b in self.lb.values()) or len(self.ab) or self.ready
             if not busy and self.waiting and realtime and self.waiting[0].arrival > now:
                 time.sleep(min(0.01, self.waiting[0].arrival - now))
                 continue
