@@ -61,6 +61,28 @@ def time_lbm(g, storage, steps, warmup, skip_solid=False, tune=True):
     return out
 
 
+def config1_pipeline():
+    """The CPU-runnable workload end to end through the drop-in API (NumPy in,
+    NumPy out, so PCIe transfers included): 128x128 disk pack (r 6,
+    porosity ~0.6), D2Q9 TRT Stokes in FP64 to 1e-8, then 5000 steps of
+    transport with a first-order surface reaction (Pe 10, Da 1)."""
+    import torch
+    import prtlb
+    g = prtlb.disk_pack((128, 128), 6.0, 0.6, seed=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    info = {}
+    v = prtlb.stokes_d2q9(g, tol=1e-8, info=info)
+    t1 = time.perf_counter()
+    ainfo = {}
+    out, tn = prtlb.solve_adr(g, v, 10.0, 0.0, 5, n_species=1, steps=5000, info=ainfo,
+                              chem=prtlb.chemistry.first_order(True), da_abio=1.0)
+    t2 = time.perf_counter()
+    return dict(flow_s=round(t1 - t0, 3), flow_iters=info["flow_iters"],
+                permeability=info["permeability"], transport_s=round(t2 - t1, 3),
+                transport_steps=ainfo["steps"], total_s=round(t2 - t0, 3))
+
+
 def time_fluid_list(g, storage, steps, warmup):
     """One geometry as a fluid-list member (compact populations + link
     table): time per step, fluid-node updates per second, and bandwidth
@@ -261,7 +283,7 @@ def campaign_study(n_geom=64, edge=(48, 160), seed=0, adr_steps=None, max_steps=
             "adr_steps_mean": float(np.mean([rs[k]["adr_steps"] for k in rs]))}
 
 
-def serve_study(n_req=48, rate=4.0, seed=0, budget=200e9, max_steps=100000):
+def serve_study(n_req=48, rate=4.0, seed=0, budget=200e9, max_steps=100000, transport_cap=None):
     """Serving: Poisson arrivals (rate per s) of solve requests, 30% D2Q9
     (256^2..1024^2 disk packs), 70% D3Q19 (64^3..256^3 sphere packs), Pe and
     Da log-uniform, reversible sorption; HBM budget admission; continuous
@@ -272,11 +294,11 @@ def serve_study(n_req=48, rate=4.0, seed=0, budget=200e9, max_steps=100000):
     reqs = serve.poisson_requests(n_req, rate=rate, seed=seed)
     log("serve geometry %.1f s" % (time.perf_counter() - t0))
     t0 = time.perf_counter()
-    done = serve.Server(hbm_budget=budget, max_steps=max_steps).run(reqs, realtime=True,
-                                                                     progress=log)
+    done = serve.Server(hbm_budget=budget, max_steps=max_steps,
+                        transport_cap=transport_cap).run(reqs, realtime=True, progress=log)
     wall = time.perf_counter() - t0
     out = serve.latency_summary(done, wall)
-    out.update(rate_per_s=rate, hbm_budget_GB=budget / 1e9,
+    out.update(rate_per_s=rate, hbm_budget_GB=budget / 1e9, transport_cap=transport_cap,
                n_2d=sum(1 for r in done if r.g.ndim == 2),
                adr_steps_mean=float(sum(r.result["info"]["steps"] for r in done) / len(done)))
     return out
@@ -300,6 +322,8 @@ def main():
     ap.add_argument("--serve", type=int, default=0,
                     help="serving study with this many Poisson requests")
     ap.add_argument("--adr-steps", type=int, default=None)
+    ap.add_argument("--transport-cap", type=float, default=None,
+                    help="serving: pore voxels allowed in the transport batch at once")
     ap.add_argument("--adr-max-steps", type=int, default=100000,
                     help="transport step cap of the campaign and serving studies")
     ap.add_argument("--adr-only", action="store_true",
@@ -335,7 +359,8 @@ def main():
               flush=True)
         return
     if args.serve:
-        print(json.dumps({"serve": serve_study(args.serve, max_steps=args.adr_max_steps)}),
+        print(json.dumps({"serve": serve_study(args.serve, max_steps=args.adr_max_steps,
+                                               transport_cap=args.transport_cap)}),
               flush=True)
         return
 
@@ -369,6 +394,8 @@ def main():
         extras["adr_128_monod_f32"] = time_adr(g128, args.steps, args.warmup)
         log("stream copy ...")
         extras["stream_copy"] = stream_copy()
+        log("2D pipeline 128^2 ...")
+        extras["d2q9_128sq_pipeline"] = config1_pipeline()
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")

@@ -191,3 +191,27 @@ def test_device_required():
         pytest.skip("a device is present")
     with pytest.raises(prtlb.PrtError):
         prtlb.stokes_d2q9(np.full((8, 6), 2, np.uint8), nit=2)
+
+
+def test_fastdiv_identity():
+    """The multiply-high division the transport kernels use to turn a flat
+    index into (x, y, z): with l = ceil(log2 d) and
+    m = floor(2^32 (2^l - d) / d) + 1, (mulhi(n, m) + n) >> l == n // d for
+    every 0 <= n < 2^31 (Granlund & Montgomery 1994).  Checked on edge
+    numerators and random ones for the divisors grids produce."""
+    rng = np.random.default_rng(0)
+    divisors = list(range(1, 600)) + [1 << k for k in range(1, 21)] + \
+        [(1 << k) - 1 for k in range(2, 21)] + [160 * 160, 256 * 256, 384 * 384, 1024 * 1024, 65535]
+    edge = np.array([0, 1, 2, (1 << 31) - 1, (1 << 31) - 2, (1 << 30), (1 << 30) - 1], np.uint64)
+    for d in divisors:
+        l = 0
+        while (1 << l) < d:
+            l += 1
+        m = ((((1 << l) - d) << 32) // d + 1) & 0xFFFFFFFF
+        assert m < (1 << 32)
+        n = np.concatenate([edge, rng.integers(0, 1 << 31, 2000).astype(np.uint64),
+                            (np.arange(1, 50, dtype=np.uint64) * d) - 1,
+                            np.arange(1, 50, dtype=np.uint64) * d])
+        n = n[n < (1 << 31)]
+        q = (((n * np.uint64(m)) >> np.uint64(32)) + n) >> np.uint64(l)
+        assert np.array_equal(q, n // np.uint64(d)), d
