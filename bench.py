@@ -190,6 +190,61 @@ def cpu_baseline(budget_s=15.0):
                        "(%.1f s) on %s" % (nit, dt, platform.processor() or platform.machine()))
 
 
+def _oracle_stokes_worker(args):
+    """One host process of the whole-host figure: the NumPy oracle's D3Q19
+    step on the given grid for nit iterations; returns seconds."""
+    g, nit = args
+    os.environ["OMP_NUM_THREADS"] = "1"
+    sys.path.insert(0, ROOT)
+    from oracle import prtlb_oracle as O
+    t0 = time.perf_counter()
+    O.stokes(g, nit=nit, force=2e-6, tol=0.0)
+    return time.perf_counter() - t0
+
+
+def cpu_extras(procs=None):
+    """The other host figures of the CPU comparison, each a few seconds:
+    the NumPy oracle's D2Q9 step (128^2 disk pack, 200 iterations), its
+    transport step in 2D (128^2, 200 steps) and 3D (64^3, 200 steps), one
+    core each; and the whole-host rate of the D3Q19 step with one process
+    per core (64^3, 20 iterations each), the campaign-style use of a host."""
+    import multiprocessing as mp
+    import numpy as np
+    import prtlb
+    from oracle import prtlb_oracle as O
+    out = {}
+    g2 = prtlb.disk_pack((128, 128), 6.0, 0.6, seed=0)
+    t0 = time.perf_counter()
+    v2 = O.stokes(g2, nit=200, force=2e-6, tol=0.0)
+    dt = time.perf_counter() - t0
+    out["d2q9_128sq_mlups_1core"] = round(g2.size * 200 / dt / 1e6, 3)
+    t0 = time.perf_counter()
+    O.solve_adr(g2, v2, 10.0, 0.0, 3, n_species=1, steps=200,
+                chem=O.Chemistry(names=["C"], inlet=[1.0], c_init=[0.0], d_rel=[1.0],
+                                 immobile=[False], biotic_on=False, abiotic_index=0,
+                                 abiotic_surface_only=True), da_abio=1.0)
+    dt = time.perf_counter() - t0
+    out["adr_2d_128sq_pore_voxel_steps_per_s_1core"] = round(int((g2 == 2).sum()) * 200 / dt, 1)
+    g3 = prtlb.sphere_pack((64, 64, 64), (4.0, 8.0), 0.35, seed=0)
+    v3 = O.stokes(g3, nit=20, force=2e-6, tol=0.0)
+    t0 = time.perf_counter()
+    O.solve_adr(g3, v3, 10.0, 1.0, 3, n_species=1, steps=200,
+                chem=O.Chemistry(names=["C"], inlet=[1.0], c_init=[0.0], d_rel=[1.0],
+                                 immobile=[False], ks_donor=0.5, biotic_on=True))
+    dt = time.perf_counter() - t0
+    out["adr_3d_64cube_pore_voxel_steps_per_s_1core"] = round(int((g3 == 2).sum()) * 200 / dt, 1)
+    procs = procs or min(16, os.cpu_count() or 1)
+    ctx = mp.get_context("spawn")            # fresh interpreters: no GPU state inherited
+    t0 = time.perf_counter()
+    with ctx.Pool(procs) as pool:
+        secs = pool.map(_oracle_stokes_worker, [(g3, 20)] * procs)
+    wall = time.perf_counter() - t0
+    out["d3q19_64cube_mlups_all_cores"] = round(procs * g3.size * 20 / max(secs) / 1e6, 2)
+    out["all_cores_procs"] = procs
+    out["all_cores_wall_s"] = round(wall, 2)
+    return out
+
+
 def precision_study(cases=((256, 0.25), (256, 0.45), (384, 0.25), (384, 0.45)),
                     nit=20000, tol=1e-8, adr_steps=2000):
     """Storage-precision errors against the FP64 GPU solve on seeded sphere
@@ -400,6 +455,9 @@ def main():
     if not args.no_cpu:
         log("cpu baseline ...")
         cpu = cpu_baseline()
+        if not args.no_extras:
+            log("cpu extras ...")
+            extras["cpu_extras"] = cpu_extras()
     line = {
         "metric": "D3Q19 TRT MLUPS & %HBM peak; batched solves/hr + p99 latency at fixed error",
         "value": round(main_r["mlups"], 1),

@@ -51,6 +51,10 @@ extern "C" {
   (((bz_log2) << 8) | ((by_log2) << 12) | (((npt) - 1) << 16))
 /* nontemporal population loads (dense mode, one node per thread) */
 #define PRT_LBM_NT_LOADS (1 << 18)
+/* XCD-contiguous block order (dense mode, one node per thread): each of the
+ * 8 XCDs runs a contiguous slab of blocks, so lines shared by neighbouring
+ * blocks meet in one L2; results are unchanged */
+#define PRT_LBM_XCD (1 << 19)
 /* prt_lbm_batch_step: the members are fluid lists (prt_lbm_segment.link) */
 #define PRT_LBM_SPARSE (1 << 1)
 

@@ -38,7 +38,7 @@ template <typename R, int ND, int NSM>
 __global__ __launch_bounds__(256) void adr_list_kernel(AdrArgs<R> a, const int32_t *__restrict__ pidx,
                                                       const uint32_t *__restrict__ word,
                                                       const R *__restrict__ face, uint32_t npore) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (k >= npore) return;
   R cnew[NSM];
   adr_entry<R, ND, NSM>(a, pidx, word, face, k, npore, cnew);

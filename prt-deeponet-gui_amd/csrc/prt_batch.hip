@@ -57,13 +57,14 @@ __global__ __launch_bounds__(256) void lbm_batch_kernel(const SegLbm *__restrict
                                                         const int *__restrict__ blk_seg, int parity,
                                                         double *partials) {
   __shared__ double lds[16];
-  const int s = __builtin_amdgcn_readfirstlane(blk_seg[blockIdx.x]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int s = __builtin_amdgcn_readfirstlane(blk_seg[blk]);
   const SegLbm &d = segs[s];
   LbmArgs a = d.a;
   const bool newest_a = ((parity ^ d.phase) & 1) == 0;
   a.src = newest_a ? d.fa : d.fb;
   a.dst = newest_a ? d.fb : d.fa;
-  const int n = (int)(blockIdx.x - d.blk0) * 256 + threadIdx.x;
+  const int n = (int)(blk - d.blk0) * 256 + threadIdx.x;
   double contrib = 0.0;
   if (n < a.N) {
     int x, y, z;
@@ -88,7 +89,7 @@ __global__ __launch_bounds__(256) void lbm_batch_kernel(const SegLbm *__restrict
   }
   if (RED) {
     const double t = block_sum(contrib, lds);
-    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+    if (threadIdx.x == 0) partials[blk] = t;
   }
 }
 
@@ -99,13 +100,14 @@ __global__ __launch_bounds__(256) void lbm_batch_sparse_kernel(const SegLbm *__r
                                                                const int *__restrict__ blk_seg, int parity,
                                                                double *partials) {
   __shared__ double lds[16];
-  const int s = __builtin_amdgcn_readfirstlane(blk_seg[blockIdx.x]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int s = __builtin_amdgcn_readfirstlane(blk_seg[blk]);
   const SegLbm &d = segs[s];
   LbmArgs a = d.a;
   const bool newest_a = ((parity ^ d.phase) & 1) == 0;
   a.src = newest_a ? d.fa : d.fb;
   a.dst = newest_a ? d.fb : d.fa;
-  const int k = (int)(blockIdx.x - d.blk0) * 256 + threadIdx.x;
+  const int k = (int)(blk - d.blk0) * 256 + threadIdx.x;
   double contrib = 0.0;
   if (k < a.N) {
     double ux;
@@ -124,7 +126,7 @@ __global__ __launch_bounds__(256) void lbm_batch_sparse_kernel(const SegLbm *__r
   }
   if (RED) {
     const double t = block_sum(contrib, lds);
-    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+    if (threadIdx.x == 0) partials[blk] = t;
   }
 }
 
@@ -133,13 +135,14 @@ __global__ __launch_bounds__(256) void adr_batch_kernel(const SegAdr *__restrict
                                                         const int *__restrict__ blk_seg, int parity,
                                                         double *partials) {
   __shared__ double lds[16];
-  const int s = __builtin_amdgcn_readfirstlane(blk_seg[blockIdx.x]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int s = __builtin_amdgcn_readfirstlane(blk_seg[blk]);
   const SegAdr &d = segs[s];
   AdrArgs<float> a = d.a;
   const bool newest_a = ((parity ^ d.phase) & 1) == 0;
   a.cin = newest_a ? d.ca : d.cb;
   a.cout = newest_a ? d.cb : d.ca;
-  const uint32_t k = (uint32_t)(blockIdx.x - d.blk0) * 256 + threadIdx.x;
+  const uint32_t k = (blk - (uint32_t)d.blk0) * 256 + threadIdx.x;
   float cnew[NSM];
   bool pore = false;
   if (k < (uint32_t)d.npts)
@@ -150,7 +153,7 @@ __global__ __launch_bounds__(256) void adr_batch_kernel(const SegAdr *__restrict
 #pragma unroll
     for (int j = 0; j < NSM; ++j) {
       const double t = block_sum((pore && j < a.ns) ? (double)cnew[j] : 0.0, lds);
-      if (threadIdx.x == 0) partials[(size_t)blockIdx.x * NSM + j] = t;
+      if (threadIdx.x == 0) partials[(size_t)blk * NSM + j] = t;
       __syncthreads();
     }
   }

@@ -50,6 +50,18 @@ __device__ inline int fastdiv(int n, const FastDiv &f) {
   return (int)((__umulhi(u, f.m) + u) >> f.l);
 }
 
+// Workgroups are dispatched to the device's 8 XCDs round-robin (linear
+// workgroup b to XCD b % 8), and each XCD has its own L2.  xcd_block(b, nb)
+// renumbers the nb workgroups so that XCD x runs one contiguous range of
+// logical blocks: data shared by neighbouring blocks (stencil neighbours,
+// cache lines split between blocks) then meets in one L2 instead of being
+// fetched by several.  A bijection of [0, nb), so results do not depend on it.
+constexpr uint32_t kXcds = 8;
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  const uint32_t q = nb / kXcds, r = nb % kXcds, x = b % kXcds, slot = b / kXcds;
+  return x * q + (x < r ? x : r) + slot;
+}
+
 // Block-wide FP64 sum of one value per thread (blockDim.x * blockDim.y <=
 // 1024, a multiple of 64).  Wavefront reduction with shuffles, then LDS.
 __device__ inline double block_sum(double v, double *lds) {

@@ -27,12 +27,20 @@
 
 namespace prt {
 
-template <int Q, int ST, bool SKIP, bool RED, int NPT, bool NTL = false>
+template <int Q, int ST, bool SKIP, bool RED, int NPT, bool NTL = false, bool XCD = false>
 __global__ __launch_bounds__(512) void lbm_step_kernel(LbmArgs a) {
   __shared__ double lds[16];
-  const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y0 = (blockIdx.y * blockDim.y + threadIdx.y) * NPT;
-  const int x = blockIdx.z;
+  uint32_t bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  if constexpr (XCD) {
+    // each XCD takes a contiguous run of (z, y, x) blocks: a slab of planes
+    const uint32_t L = xcd_block(bx + gridDim.x * (by + gridDim.y * bz), gridDim.x * gridDim.y * gridDim.z);
+    bx = L % gridDim.x;
+    by = (L / gridDim.x) % gridDim.y;
+    bz = L / (gridDim.x * gridDim.y);
+  }
+  const int z = bx * blockDim.x + threadIdx.x;
+  const int y0 = (by * blockDim.y + threadIdx.y) * NPT;
+  const int x = bz;
   double contrib = 0.0;
 #pragma unroll
   for (int k = 0; k < NPT; ++k) {
@@ -61,7 +69,7 @@ __global__ __launch_bounds__(512) void lbm_step_kernel(LbmArgs a) {
   if (RED) {
     const double s = block_sum(contrib, lds);
     if (threadIdx.x == 0 && threadIdx.y == 0)
-      a.partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+      a.partials[bx + gridDim.x * (by + gridDim.y * bz)] = s;
   }
 }
 
@@ -183,6 +191,7 @@ struct Geom {
   dim3 block, grid;
   int npt;
   bool ntl;
+  bool xcd;
 };
 
 static int make_geom(int q, int nx, int ny, int nz, Geom &g, int flags = 0) {
@@ -215,6 +224,7 @@ static int make_geom(int q, int nx, int ny, int nz, Geom &g, int flags = 0) {
   int by = 256 / bz;
   g.npt = 1;
   g.ntl = (flags >> 18) & 1;
+  g.xcd = (flags >> 19) & 1;
   const int tz = (flags >> 8) & 15, ty = (flags >> 12) & 15, tn = (flags >> 16) & 3;
   if (tz) {
     bz = 1 << tz;
@@ -275,7 +285,16 @@ static void launch_step_n(const Geom &g, const LbmArgs &a, bool skip, bool red, 
 
 template <int Q, int ST>
 static void launch_step_t(const Geom &g, const LbmArgs &a, bool skip, bool red, hipStream_t st) {
-  if (g.ntl && !skip && g.npt == 1) {
+  if (g.xcd && !skip && g.npt == 1) {
+    // XCD-contiguous block order (dense mode, one node per thread)
+    if (g.ntl) {
+      if (red) lbm_step_kernel<Q, ST, false, true, 1, true, true><<<g.grid, g.block, 0, st>>>(a);
+      else lbm_step_kernel<Q, ST, false, false, 1, true, true><<<g.grid, g.block, 0, st>>>(a);
+    } else {
+      if (red) lbm_step_kernel<Q, ST, false, true, 1, false, true><<<g.grid, g.block, 0, st>>>(a);
+      else lbm_step_kernel<Q, ST, false, false, 1, false, true><<<g.grid, g.block, 0, st>>>(a);
+    }
+  } else if (g.ntl && !skip && g.npt == 1) {
     // nontemporal population loads (dense mode, one node per thread)
     if (red) lbm_step_kernel<Q, ST, false, true, 1, true><<<g.grid, g.block, 0, st>>>(a);
     else lbm_step_kernel<Q, ST, false, false, 1, true><<<g.grid, g.block, 0, st>>>(a);

@@ -57,7 +57,8 @@ template <int Q, int ST, bool RED>
 __global__ __launch_bounds__(kSparseBlock) void lbm_sparse_kernel(LbmArgs a, const int32_t *__restrict__ link,
                                                                  long long lk) {
   __shared__ double lds[16];
-  const int k = blockIdx.x * kSparseBlock + threadIdx.x;
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int k = (int)blk * kSparseBlock + threadIdx.x;
   double contrib = 0.0;
   if (k < a.N) {
     double ux;
@@ -76,7 +77,7 @@ __global__ __launch_bounds__(kSparseBlock) void lbm_sparse_kernel(LbmArgs a, con
   }
   if (RED) {
     const double t = block_sum(contrib, lds);
-    if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
+    if (threadIdx.x == 0) a.partials[blk] = t;
   }
 }
 

@@ -19,11 +19,14 @@ PRT_LBM_SKIP_SOLID = 1
 
 PRT_LBM_NT_LOADS = 1 << 18
 PRT_LBM_SPARSE = 1 << 1
+PRT_LBM_XCD = 1 << 19
 
 
-def lbm_tune(bz_log2, by_log2, npt, nt_loads=False):
-    """PRT_LBM_TUNE (| PRT_LBM_NT_LOADS): launch bits for prt_lbm_step / _run."""
-    return (bz_log2 << 8) | (by_log2 << 12) | ((npt - 1) << 16) | (PRT_LBM_NT_LOADS if nt_loads else 0)
+def lbm_tune(bz_log2, by_log2, npt, nt_loads=False, xcd=False):
+    """PRT_LBM_TUNE (| PRT_LBM_NT_LOADS | PRT_LBM_XCD): launch bits for
+    prt_lbm_step / _run."""
+    return ((bz_log2 << 8) | (by_log2 << 12) | ((npt - 1) << 16) | (PRT_LBM_NT_LOADS if nt_loads else 0)
+            | (PRT_LBM_XCD if xcd else 0))
 
 
 class PrtError(RuntimeError):

@@ -15,11 +15,14 @@ from ._lib import lbm_tune
 
 _CACHE = {}
 
-# (log2 threads along z, log2 threads along y, nodes per thread, nontemporal loads)
-CANDIDATES = [(6, 0, 1, 0), (6, 1, 1, 0), (6, 2, 1, 0), (6, 3, 1, 0), (7, 0, 1, 0), (7, 1, 1, 0),
-              (7, 2, 1, 0), (8, 0, 1, 0), (8, 1, 1, 0), (9, 0, 1, 0), (6, 2, 2, 0), (7, 1, 2, 0),
-              (8, 0, 2, 0), (6, 3, 2, 0), (6, 2, 1, 1), (7, 1, 1, 1), (7, 0, 1, 1), (8, 0, 1, 1),
-              (8, 1, 1, 1)]
+# (log2 threads along z, log2 threads along y, nodes per thread, nontemporal
+# loads, XCD-contiguous block order)
+CANDIDATES = [(6, 0, 1, 0, 0), (6, 1, 1, 0, 0), (6, 2, 1, 0, 0), (6, 3, 1, 0, 0), (7, 0, 1, 0, 0),
+              (7, 1, 1, 0, 0), (7, 2, 1, 0, 0), (8, 0, 1, 0, 0), (8, 1, 1, 0, 0), (9, 0, 1, 0, 0),
+              (6, 2, 2, 0, 0), (7, 1, 2, 0, 0), (8, 0, 2, 0, 0), (6, 3, 2, 0, 0), (6, 2, 1, 1, 0),
+              (7, 1, 1, 1, 0), (7, 0, 1, 1, 0), (8, 0, 1, 1, 0), (8, 1, 1, 1, 0),
+              (6, 2, 1, 1, 1), (7, 1, 1, 1, 1), (7, 0, 1, 1, 1), (8, 0, 1, 1, 1), (8, 1, 1, 1, 1),
+              (6, 2, 1, 0, 1), (7, 1, 1, 0, 1), (8, 0, 1, 0, 1)]
 
 
 def _key(s):
@@ -73,12 +76,12 @@ def tune(s, steps=10, candidates=None, force=False):
         return _CACHE[key]
     table = {"default": time_shape(s, 0, steps)}
     best, best_ms = 0, table["default"]
-    for bz, by, npt, ntl in (candidates or CANDIDATES):
-        if (1 << bz) * (1 << by) > 512 or (ntl and (s.flags or npt != 1)):
+    for bz, by, npt, ntl, xcd in (candidates or CANDIDATES):
+        if (1 << bz) * (1 << by) > 512 or ((ntl or xcd) and (s.flags or npt != 1)):
             continue
-        t = lbm_tune(bz, by, npt, bool(ntl))
+        t = lbm_tune(bz, by, npt, bool(ntl), bool(xcd))
         ms = time_shape(s, t, steps)
-        table["%dx%dx%d%s" % (1 << bz, 1 << by, npt, "nt" if ntl else "")] = ms
+        table["%dx%dx%d%s%s" % (1 << bz, 1 << by, npt, "nt" if ntl else "", "x" if xcd else "")] = ms
         if ms < best_ms:
             best, best_ms = t, ms
     s.tune = best

@@ -179,3 +179,27 @@ def test_graph_replays_match_eager():
     v1 = prtlb.stokes_d2q9(g, nit=5000, tol=1e-8, info=i1, graph=True)
     v2 = prtlb.stokes_d2q9(g, nit=5000, tol=1e-8, info=i2, graph=False)
     assert np.array_equal(v1, v2) and i1["flow_iters"] == i2["flow_iters"]
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp32", "fp16"])
+def test_launch_shapes_give_identical_bits(storage):
+    """Every launch shape the autotuner may pick (block shape, nodes per
+    thread, nontemporal loads, XCD-contiguous block order) computes the same
+    populations and the same |u_x| partial sums as the default shape."""
+    from prtlb import _lib
+    from prtlb.autotune import CANDIDATES
+    from prtlb.stokes import StokesSolver
+    g = prtlb.sphere_pack((40, 36, 130), (2.0, 5.0), 0.4, seed=4)
+    ref = StokesSolver(g, force=2e-5, storage=storage)
+    ref.run(57, reduce=True)
+    f_ref = ref.f[ref.latest].cpu().numpy()
+    m_ref = ref.mean_abs_ux()
+    for bz, by, npt, ntl, xcd in CANDIDATES:
+        if (1 << bz) * (1 << by) > 512:
+            continue
+        S = StokesSolver(g, force=2e-5, storage=storage)
+        S.tune = _lib.lbm_tune(bz, by, npt, bool(ntl), bool(xcd))
+        S.run(57, reduce=True)
+        assert np.array_equal(S.f[S.latest].cpu().numpy().view(np.uint8), f_ref.view(np.uint8)), \
+            (bz, by, npt, ntl, xcd)
+        assert S.mean_abs_ux() == pytest.approx(m_ref, rel=1e-12)
