@@ -34,15 +34,20 @@ __global__ __launch_bounds__(256) void adr_step_kernel(AdrArgs<R> a) {
 
 // One thread per entry of a pore list (no lane of a wave is spent on solid),
 // geometry from the pore table.  Same update code, so the same bits.
-template <typename R, int ND, int NSM>
+template <typename R, int ND, int NSM, bool NTT>
 __global__ __launch_bounds__(256) void adr_list_kernel(AdrArgs<R> a, const int32_t *__restrict__ pidx,
                                                       const uint32_t *__restrict__ word,
                                                       const R *__restrict__ face, uint32_t npore) {
   const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (k >= npore) return;
   R cnew[NSM];
-  adr_entry<R, ND, NSM>(a, pidx, word, face, k, npore, cnew);
+  adr_entry<R, ND, NSM, NTT>(a, pidx, word, face, k, npore, cnew);
 }
+
+// Pore lists at least this long stream their table with nontemporal loads:
+// the table (32 B per pore voxel in float32) then exceeds what the 256 MB
+// last-level cache keeps between steps next to the concentration arrays.
+constexpr uint32_t kNtTableEntries = 1u << 22;
 
 // Pore table of a list: the word and face velocities of every entry, or
 // kPoreWordNone (and zero faces) for an entry that is not a pore voxel.
@@ -102,14 +107,21 @@ static int launch(const prt_adr_params *p, const AdrArgs<R> &a, dim3 g, dim3 b, 
   return launch_status("prt_adr_step launch");
 }
 
+template <typename R, int ND, bool NTT>
+static void launch_list_ndt(const AdrArgs<R> &a, const int32_t *pidx, const uint32_t *word, const R *face,
+                            uint32_t npore, hipStream_t st) {
+  const dim3 g((npore + 255) / 256), b(256);
+  if (a.ns == 1) adr_list_kernel<R, ND, 1, NTT><<<g, b, 0, st>>>(a, pidx, word, face, npore);
+  else if (a.ns == 2) adr_list_kernel<R, ND, 2, NTT><<<g, b, 0, st>>>(a, pidx, word, face, npore);
+  else if (a.ns <= 4) adr_list_kernel<R, ND, 4, NTT><<<g, b, 0, st>>>(a, pidx, word, face, npore);
+  else adr_list_kernel<R, ND, 8, NTT><<<g, b, 0, st>>>(a, pidx, word, face, npore);
+}
+
 template <typename R, int ND>
 static void launch_list_nd(const AdrArgs<R> &a, const int32_t *pidx, const uint32_t *word, const R *face,
                            uint32_t npore, hipStream_t st) {
-  const dim3 g((npore + 255) / 256), b(256);
-  if (a.ns == 1) adr_list_kernel<R, ND, 1><<<g, b, 0, st>>>(a, pidx, word, face, npore);
-  else if (a.ns == 2) adr_list_kernel<R, ND, 2><<<g, b, 0, st>>>(a, pidx, word, face, npore);
-  else if (a.ns <= 4) adr_list_kernel<R, ND, 4><<<g, b, 0, st>>>(a, pidx, word, face, npore);
-  else adr_list_kernel<R, ND, 8><<<g, b, 0, st>>>(a, pidx, word, face, npore);
+  if (npore >= kNtTableEntries) launch_list_ndt<R, ND, true>(a, pidx, word, face, npore, st);
+  else launch_list_ndt<R, ND, false>(a, pidx, word, face, npore, st);
 }
 
 // Arguments of a list launch: parameters, grid, pore list and table.

@@ -64,6 +64,14 @@ template <typename T>
 __device__ inline T ld32(const T *base, uint32_t i) {
   return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + i * (uint32_t)sizeof(T));
 }
+// The same for data read once per step and never again by a neighbour (the
+// pore list and table of a large grid): a nontemporal load, so it does not
+// displace the concentration lines the stencil re-reads.
+template <typename T>
+__device__ inline T ldnt32(const T *base, uint32_t i) {
+  return __builtin_nontemporal_load(
+      reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + i * (uint32_t)sizeof(T)));
+}
 template <typename T>
 __device__ inline void st32(T *base, uint32_t i, T v) {
   *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + i * (uint32_t)sizeof(T)) = v;
@@ -160,18 +168,26 @@ using vec4 = R __attribute__((ext_vector_type(4)));
 template <typename R>
 using vec2 = R __attribute__((ext_vector_type(2)));
 
-template <typename R, int ND>
+// Load of a pore-table entry: nontemporal (NTT) for lists too large to
+// stay in the last-level cache between steps, ordinary otherwise.
+template <bool NTT, typename T>
+__device__ inline T ldtab(const T *base, uint32_t i) {
+  if constexpr (NTT) return ldnt32(base, i);
+  else return ld32(base, i);
+}
+
+template <typename R, int ND, bool NTT>
 __device__ inline void geom_table(const AdrArgs<R> &a, uint32_t n, uint32_t w, const R *face, uint32_t k,
                                   uint32_t n_pore, VoxGeom<R, ND> &g) {
   g.w = w;
   geom_index<R, ND>(a, n, g);
-  const vec4<R> f01 = ld32(reinterpret_cast<const vec4<R> *>(face), k);
+  const vec4<R> f01 = ldtab<NTT>(reinterpret_cast<const vec4<R> *>(face), k);
   g.u_lo[0] = f01.x;
   g.u_hi[0] = f01.y;
   g.u_lo[1] = f01.z;
   g.u_hi[1] = f01.w;
   if (ND == 3) {
-    const vec2<R> f2 = ld32(reinterpret_cast<const vec2<R> *>(face + (size_t)4 * n_pore), k);
+    const vec2<R> f2 = ldtab<NTT>(reinterpret_cast<const vec2<R> *>(face + (size_t)4 * n_pore), k);
     g.u_lo[ND - 1] = f2.x;
     g.u_hi[ND - 1] = f2.y;
   }
@@ -371,14 +387,14 @@ __device__ inline bool adr_voxel(const AdrArgs<R> &a, uint32_t n, int x, int y, 
 
 // List entry k from the pore table.  Returns false for an entry marked not
 // pore (kPoreWordNone).
-template <typename R, int ND, int NSM>
+template <typename R, int ND, int NSM, bool NTT = false>
 __device__ inline bool adr_entry(const AdrArgs<R> &a, const int32_t *pidx, const uint32_t *word, const R *face,
                                  uint32_t k, uint32_t n_pore, R *cnew) {
-  const uint32_t w = ld32(word, k);
+  const uint32_t w = ldtab<NTT>(word, k);
   if (w == kPoreWordNone) return false;
-  const uint32_t n = (uint32_t)ld32(pidx, k);
+  const uint32_t n = (uint32_t)ldtab<NTT>(pidx, k);
   VoxGeom<R, ND> g;
-  geom_table<R, ND>(a, n, w, face, k, n_pore, g);
+  geom_table<R, ND, NTT>(a, n, w, face, k, n_pore, g);
   adr_update<R, ND, NSM>(a, n, g, cnew);
   return true;
 }

@@ -12,7 +12,6 @@ the host; the per-species pore means they read are NumPy-ordered float32
 sums evaluated on the device, so an auto-stopped run stops at the same step.
 """
 import ctypes
-import math
 
 import numpy as np
 import torch

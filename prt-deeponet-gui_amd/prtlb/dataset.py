@@ -18,12 +18,10 @@ build, so the container is an .npz archive whose keys are the HDF5 paths
 import time
 
 import numpy as np
-import torch
 
 from . import campaign as C
-from . import chemistry as chemmod
 from .batch import AdrBatch, LbmBatch
-from .geometry import PORE, edt_device, geodesic_device, sphere_pack
+from .geometry import edt_device, geodesic_device, sphere_pack
 from .physics import check_physics_device
 
 PARAM_NAMES = ("pe", "da", "da_abio")

@@ -19,7 +19,6 @@ import time
 from collections import deque
 
 import numpy as np
-import torch
 
 from . import campaign as C
 from .batch import AdrBatch, LbmBatch, adr_footprint, lbm_footprint

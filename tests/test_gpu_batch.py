@@ -55,7 +55,6 @@ def test_fluid_list_step_equals_dense_fluid_only(q, storage):
     """The fluid-list kernels against the dense fluid-only kernels: the
     compact populations equal the dense ones at every fluid node, and the
     velocity fields are equal, bit for bit."""
-    import ctypes
     import torch
     from prtlb import _lib
     from prtlb.device import ptr, stream_ptr

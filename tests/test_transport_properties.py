@@ -135,12 +135,14 @@ def test_irreversible_sorption_conserves_total(backend, rock):
 def test_gpu_equals_oracle_on_the_property_cases(rock):
     """The GPU field equals the oracle's bit for bit on these cases too."""
     g, v = rock
-    for kw in (dict(names=["C"], inlet=[1.0], c_init=[0.0], d_rel=[1.0], immobile=[False],
-                    biotic_on=False),
-               dict(names=["C", "S"], inlet=[1.0, NAN], c_init=[0.0, 0.0], d_rel=[1.0, 0.0],
-                    immobile=[False, True], biotic_on=False, abiotic_index=0,
-                    abiotic_product_index=1, abiotic_surface_only=True)):
+    for da_abio, kw in ((0.0, dict(names=["C"], inlet=[1.0], c_init=[0.0], d_rel=[1.0],
+                                   immobile=[False], biotic_on=False)),
+                        (1.0, dict(names=["C", "S"], inlet=[1.0, NAN], c_init=[0.0, 0.0],
+                                   d_rel=[1.0, 0.0], immobile=[False, True], biotic_on=False,
+                                   abiotic_index=0, abiotic_product_index=1,
+                                   abiotic_surface_only=True))):
         ns = len(kw["names"])
-        a, _, _ = _evolve("oracle", g, v, 50.0, 0.0, ns, _chem("oracle", **kw), 200, da_abio=1.0)
-        b, _, _ = _evolve("gpu", g, v, 50.0, 0.0, ns, _chem("gpu", **kw), 200, da_abio=1.0)
+        a, _, _ = _evolve("oracle", g, v, 50.0, 0.0, ns, _chem("oracle", **kw), 200,
+                          da_abio=da_abio)
+        b, _, _ = _evolve("gpu", g, v, 50.0, 0.0, ns, _chem("gpu", **kw), 200, da_abio=da_abio)
         assert np.array_equal(a, b)
