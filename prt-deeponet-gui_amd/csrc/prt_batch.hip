@@ -130,12 +130,14 @@ __global__ __launch_bounds__(256) void lbm_batch_sparse_kernel(const SegLbm *__r
   }
 }
 
-template <int NSM, bool RED, bool NTT>
+template <int NSM, bool RED>
 __global__ __launch_bounds__(256) void adr_batch_kernel(const SegAdr *__restrict__ segs,
                                                         const int *__restrict__ blk_seg, int parity,
                                                         double *partials) {
   __shared__ double lds[16];
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  // round-robin block order here: measured faster than the XCD-contiguous
+  // order for batches of many small members
+  const uint32_t blk = blockIdx.x;
   const int s = __builtin_amdgcn_readfirstlane(blk_seg[blk]);
   const SegAdr &d = segs[s];
   AdrArgs<float> a = d.a;
@@ -146,8 +148,8 @@ __global__ __launch_bounds__(256) void adr_batch_kernel(const SegAdr *__restrict
   float cnew[NSM];
   bool pore = false;
   if (k < (uint32_t)d.npts)
-    pore = d.nd == 3 ? adr_entry<float, 3, NSM, NTT>(a, d.pidx, d.word, d.face, k, (uint32_t)d.npts, cnew)
-                     : adr_entry<float, 2, NSM, NTT>(a, d.pidx, d.word, d.face, k, (uint32_t)d.npts, cnew);
+    pore = d.nd == 3 ? adr_entry<float, 3, NSM>(a, d.pidx, d.word, d.face, k, (uint32_t)d.npts, cnew)
+                     : adr_entry<float, 2, NSM>(a, d.pidx, d.word, d.face, k, (uint32_t)d.npts, cnew);
   if (RED) {
     // per-block pore sums of every chemical (FP64), NSM slots per block
 #pragma unroll
@@ -345,15 +347,10 @@ extern "C" int prt_adr_batch_step(const void *desc_dev, const int32_t *blk_seg_d
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid((unsigned)n_blocks), block(256);
   const bool red = partials != nullptr;
-  // batches whose lists total at least 2^22 entries stream their tables
-  // with nontemporal loads (the single-geometry launch's rule)
-  const bool ntt = n_blocks * 256 >= (1LL << 22);
-#define PRT_AB(NN)                                                                                   \
-  do {                                                                                               \
-    if (red && ntt) adr_batch_kernel<NN, true, true><<<grid, block, 0, st>>>(d, blk_seg_dev, parity, partials); \
-    else if (red) adr_batch_kernel<NN, true, false><<<grid, block, 0, st>>>(d, blk_seg_dev, parity, partials);  \
-    else if (ntt) adr_batch_kernel<NN, false, true><<<grid, block, 0, st>>>(d, blk_seg_dev, parity, partials);  \
-    else adr_batch_kernel<NN, false, false><<<grid, block, 0, st>>>(d, blk_seg_dev, parity, partials);          \
+#define PRT_AB(NN)                                                                            \
+  do {                                                                                        \
+    if (red) adr_batch_kernel<NN, true><<<grid, block, 0, st>>>(d, blk_seg_dev, parity, partials);   \
+    else adr_batch_kernel<NN, false><<<grid, block, 0, st>>>(d, blk_seg_dev, parity, partials);      \
   } while (0)
   if (max_species == 1) PRT_AB(1);
   else if (max_species == 2) PRT_AB(2);
