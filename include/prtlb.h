@@ -100,6 +100,10 @@ int prt_lbm_step(int q, int storage, int nx, int ny, int nz,
 /* Length a partials buffer needs for this grid (any launch shape). */
 int64_t prt_lbm_partials_count(int q, int nx, int ny, int nz);
 
+/* Partials one launch with these flags writes (indices [0, blocks)); the
+ * velocity kernel launches the flags = 0 shape.  Sum exactly these. */
+int64_t prt_lbm_launch_blocks(int q, int nx, int ny, int nz, int flags);
+
 /* `steps` fused iterations alternating f_a -> f_b -> f_a ...; the last one
  * writes f_b if steps is odd, else f_a.  Partials (if non-NULL) are
  * produced by the last iteration only. */
@@ -410,6 +414,52 @@ int prt_check_moments(const float *out, int n_t, int ns, const uint8_t *pore,
 int prt_check_copy_residual(const float *out, int n_t, int ns, int j, float sc,
                             const uint8_t *pore, int64_t n_vox, int blocks,
                             float *part, void *stream);
+
+/* ---------------------------------------------------------- solver handles */
+/* Handles over the stateless calls above, for hosts that prefer an object
+ * (the create / init / step_n / reduce / velocity / destroy set a non-Python
+ * caller of the Stokes and transport stages binds).  A handle keeps the
+ * caller's device pointers, grid, rates, stream and which buffer is newest;
+ * it owns no device memory, is bound to one stream and is not shared between
+ * threads.  Same arithmetic, same bits as the calls it wraps. */
+typedef struct prt_lbm_solver prt_lbm_solver;
+typedef struct prt_adr_solver prt_adr_solver;
+
+#define PRT_REDUCE_MEAN_ABS_UX 0 /* mean |u_x| over fluid nodes after the last
+                                    step_n (the FlowConvergence quantity) */
+#define PRT_REDUCE_SUM_UX 1      /* sum of u_x over every node (Darcy) */
+
+/* omega- of the TRT model: 1 / (magic / (1/omega+ - 1/2) + 1/2),
+ * 3D/tools/prtlb_3d.py:130-139; NaN (and an error message) for tau <= 1/2. */
+double prt_trt_minus(double omega_p, double magic);
+
+/* partials: prt_lbm_partials_count() doubles; scalar: 1 double (device);
+ * n_fluid: the number of fluid nodes (for the mean). */
+int prt_lbm_solver_create(int q, int storage, int nx, int ny, int nz,
+                          const uint8_t *solid, long long n_fluid, void *f_a,
+                          void *f_b, double *partials, double *scalar,
+                          double tau, double magic, double force,
+                          double fp16_scale, int flags, void *stream,
+                          prt_lbm_solver **out);
+int prt_lbm_solver_init(prt_lbm_solver *h);          /* rest state, both buffers */
+int prt_lbm_solver_step_n(prt_lbm_solver *h, int n); /* partials on the last */
+/* Synchronises the stream; the value lands in *out_host. */
+int prt_lbm_solver_reduce(prt_lbm_solver *h, int what, double *out_host);
+int prt_lbm_solver_velocity(prt_lbm_solver *h, double *v64, float *v32);
+void *prt_lbm_solver_newest(prt_lbm_solver *h);     /* newest population buffer */
+int prt_lbm_solver_destroy(prt_lbm_solver *h);
+
+/* A transport solve over a pore list and its table (prt_adr_pore_table);
+ * the parameter block is copied into the handle. */
+int prt_adr_solver_create(const prt_adr_params *p, int nx, int ny, int nz,
+                          const int32_t *pore_idx, const uint32_t *word,
+                          const float *face, long long n_pore, float *c_a,
+                          float *c_b, void *stream, prt_adr_solver **out);
+int prt_adr_solver_step_n(prt_adr_solver *h, int n);
+float *prt_adr_solver_newest(prt_adr_solver *h);    /* newest (ns, n) field */
+/* Device-to-device copy of the newest field into dst (ns * n floats). */
+int prt_adr_solver_snapshot(prt_adr_solver *h, float *dst);
+int prt_adr_solver_destroy(prt_adr_solver *h);
 
 /* --------------------------------------------------------- measurement */
 

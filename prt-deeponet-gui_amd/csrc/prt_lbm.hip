@@ -359,6 +359,14 @@ extern "C" int prt_lbm_fill_rest(int q, int storage, int64_t n_nodes, void *f, v
   return launch_status("prt_lbm_fill_rest launch");
 }
 
+extern "C" int64_t prt_lbm_launch_blocks(int q, int nx, int ny, int nz, int flags) {
+  // partials a launch with these flags writes: one per block, indices
+  // [0, blocks); the velocity kernel launches the flags = 0 shape
+  Geom g;
+  if (make_geom(q, nx, ny, nz, g, flags) != PRT_OK) return -1;
+  return (int64_t)g.grid.x * g.grid.y * g.grid.z;
+}
+
 extern "C" int64_t prt_lbm_partials_count(int q, int nx, int ny, int nz) {
   // the largest block count any tuning of this grid launches (64 threads)
   Geom g;

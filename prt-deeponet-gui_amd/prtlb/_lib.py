@@ -92,6 +92,22 @@ _SIGS = {
     "prt_adr_step_f64": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _P]),
     "prt_adr_run": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _I, _P]),
     "prt_lbm_link_stride": (_I64, [_I64]),
+    "prt_lbm_launch_blocks": (_I64, [_I, _I, _I, _I, _I]),
+    "prt_trt_minus": (_D, [_D, _D]),
+    "prt_lbm_solver_create": (_I, [_I, _I, _I, _I, _I, _P, ctypes.c_longlong, _P, _P, _P, _P, _D, _D, _D,
+                                   _D, _I, _P, _P]),
+    "prt_lbm_solver_init": (_I, [_P]),
+    "prt_lbm_solver_step_n": (_I, [_P, _I]),
+    "prt_lbm_solver_reduce": (_I, [_P, _I, _P]),
+    "prt_lbm_solver_velocity": (_I, [_P, _P, _P]),
+    "prt_lbm_solver_newest": (_P, [_P]),
+    "prt_lbm_solver_destroy": (_I, [_P]),
+    "prt_adr_solver_create": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, ctypes.c_longlong,
+                                   _P, _P, _P, _P]),
+    "prt_adr_solver_step_n": (_I, [_P, _I]),
+    "prt_adr_solver_newest": (_P, [_P]),
+    "prt_adr_solver_snapshot": (_I, [_P, _P]),
+    "prt_adr_solver_destroy": (_I, [_P]),
     "prt_lbm_link_table": (_I, [_I, _I, _I, _I, _P, _I64, _P, _P, _P]),
     "prt_lbm_sparse_fill_rest": (_I, [_I, _I, _I64, _P, _P]),
     "prt_lbm_sparse_partials_count": (_I, [_I64]),

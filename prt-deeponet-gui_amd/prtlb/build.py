@@ -27,6 +27,7 @@ SOURCES = {
     "prt_batch.hip": [],
     "prt_geom.hip": ["-ffp-contract=off"],
     "prt_check.hip": ["-ffp-contract=off"],
+    "prt_handle.hip": [],
 }
 
 

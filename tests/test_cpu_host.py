@@ -215,3 +215,29 @@ def test_fastdiv_identity():
         n = n[n < (1 << 31)]
         q = (((n * np.uint64(m)) >> np.uint64(32)) + n) >> np.uint64(l)
         assert np.array_equal(q, n // np.uint64(d)), d
+
+
+def test_handle_api_host_side():
+    """The handle API's host-only parts: omega- agrees with the host rule,
+    tau <= 1/2 is refused, and handles are created and destroyed without a
+    device (no device call happens before a step)."""
+    import ctypes
+    L = _lib.lib()
+    for tau in (0.55, 0.8, 1.0, 2.0):
+        assert L.prt_trt_minus(1.0 / tau, 3.0 / 16.0) == pytest.approx(prtlb.trt_minus(1.0 / tau), rel=1e-15)
+    assert np.isnan(L.prt_trt_minus(1.0 / 0.5, 3.0 / 16.0))
+    buf = (ctypes.c_double * 8)()
+    p = ctypes.addressof(buf)
+    h = ctypes.c_void_p()
+    _lib.call("prt_lbm_solver_create", 19, 0, 4, 4, 4, p, 10, p + 8, p + 16, p + 24, p + 32,
+              0.8, 3.0 / 16.0, 2e-6, 1024.0, 0, None, ctypes.byref(h))
+    assert h.value
+    assert L.prt_lbm_solver_newest(h) == p + 8
+    _lib.call("prt_lbm_solver_destroy", h)
+    with pytest.raises(_lib.PrtError):
+        _lib.call("prt_lbm_solver_create", 19, 0, 4, 4, 4, p, 10, p + 8, p + 8, p + 24, p + 32,
+                  0.8, 3.0 / 16.0, 2e-6, 1024.0, 0, None, ctypes.byref(h))     # aliased buffers
+    with pytest.raises(_lib.PrtError):
+        _lib.call("prt_lbm_solver_create", 19, 0, 4, 4, 4, p, 10, p + 8, p + 16, p + 24, p + 32,
+                  0.5, 3.0 / 16.0, 2e-6, 1024.0, 0, None, ctypes.byref(h))     # tau = 1/2
+    assert L.prt_lbm_launch_blocks(19, 8, 8, 130, 0) > 0

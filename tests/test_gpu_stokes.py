@@ -203,3 +203,63 @@ def test_launch_shapes_give_identical_bits(storage):
         assert np.array_equal(S.f[S.latest].cpu().numpy().view(np.uint8), f_ref.view(np.uint8)), \
             (bz, by, npt, ntl, xcd)
         assert S.mean_abs_ux() == pytest.approx(m_ref, rel=1e-12)
+
+
+def test_solver_handles_match_stateless_calls():
+    """The handle API (create / init / step_n / reduce / velocity / destroy)
+    gives the bits of the StokesSolver path over the same buffers, and the
+    transport handle those of TransportSolver."""
+    import ctypes
+    import torch
+    from prtlb import _lib
+    from prtlb.device import ptr, stream_ptr
+    from prtlb.stokes import StokesSolver
+    g = prtlb.sphere_pack((30, 22, 26), (2.0, 4.0), 0.45, seed=13)
+    S = StokesSolver(g, force=3e-5, storage="fp64")
+    S.run(120, reduce=True)
+    m_ref = S.mean_abs_ux()
+    v32_ref, v64_ref, sux_ref = S.velocity(f64=True)
+    L = _lib.lib()
+    fa = torch.empty_like(S.f[0])
+    fb = torch.empty_like(S.f[0])
+    part = torch.zeros(L.prt_lbm_partials_count(19, *S.dims), dtype=torch.float64, device=S.dev)
+    scal = torch.zeros(1, dtype=torch.float64, device=S.dev)
+    h = ctypes.c_void_p()
+    _lib.call("prt_lbm_solver_create", 19, 0, *S.dims, ptr(S.solid), S.n_fluid, ptr(fa), ptr(fb),
+              ptr(part), ptr(scal), 0.8, 3.0 / 16.0, 3e-5, 1024.0, 0, stream_ptr(S.dev), ctypes.byref(h))
+    try:
+        _lib.call("prt_lbm_solver_init", h)
+        _lib.call("prt_lbm_solver_step_n", h, 70)
+        _lib.call("prt_lbm_solver_step_n", h, 50)
+        mean = ctypes.c_double()
+        _lib.call("prt_lbm_solver_reduce", h, 0, ctypes.byref(mean))
+        assert mean.value == m_ref
+        v64 = torch.empty((3, S.n), dtype=torch.float64, device=S.dev)
+        _lib.call("prt_lbm_solver_velocity", h, ptr(v64), None)
+        assert np.array_equal(v64.cpu().numpy().reshape(v64_ref.shape), v64_ref.cpu().numpy())
+        sux = ctypes.c_double()
+        _lib.call("prt_lbm_solver_reduce", h, 1, ctypes.byref(sux))
+        assert sux.value == pytest.approx(sux_ref, rel=1e-12)
+        # 120 steps: the newest state is back in the first buffer, as in S
+        assert L.prt_lbm_solver_newest(h) == (fa if S.latest == 0 else fb).data_ptr()
+    finally:
+        _lib.call("prt_lbm_solver_destroy", h)
+    # transport handle against TransportSolver
+    v = prtlb.stokes_d3q19(g, nit=300, force=3e-5, tol=0.0)
+    T = prtlb.TransportSolver(g, v, 10.0, 1.0, 1, chem=prtlb.chemistry.monod(0.5))
+    c_a = T.c[0].clone()
+    c_b = T.c[1].clone()
+    T.use_graph = False
+    T.step_n(37)
+    ha = ctypes.c_void_p()
+    _lib.call("prt_adr_solver_create", ctypes.byref(T.params), *T.dims, ptr(T.pore_idx), ptr(T.word),
+              ptr(T.face), T.n_pore, ptr(c_a), ptr(c_b), stream_ptr(T.dev), ctypes.byref(ha))
+    try:
+        _lib.call("prt_adr_solver_step_n", ha, 20)
+        _lib.call("prt_adr_solver_step_n", ha, 17)
+        snap = torch.empty_like(c_a)
+        _lib.call("prt_adr_solver_snapshot", ha, ptr(snap))
+        torch.cuda.synchronize()
+        assert np.array_equal(snap.cpu().numpy(), T.current().cpu().numpy())
+    finally:
+        _lib.call("prt_adr_solver_destroy", ha)
