@@ -17,10 +17,14 @@
  *   - grids are C-ordered (x slowest, last axis fastest), exactly the NumPy
  *     layout of the reference's (nx, ny[, nz]) arrays.  A 2D grid is passed
  *     with nz = 1.
- *   - populations are structure-of-arrays: population i of node n lives at
- *     f[i * ld + n], ld = prt_lbm_pop_stride(n_nodes, storage) >= n_nodes
- *     (padded so the Q streams start at staggered HBM offsets); a population
- *     buffer holds q * ld elements.
+ *   - dense grids store populations by z-row: the q populations of row
+ *     (x, y) follow one another, each a contiguous run of zs elements along
+ *     z, so population i of node (x, y, z) lives at
+ *       f[((x * ny + y) * q + i) * zs + z],
+ *     zs = nz rounded up to whole 128-byte lines (a 2D grid is the row set
+ *     (nx, 1, ny)).  A buffer holds prt_lbm_pop_elems() elements.
+ *   - fluid lists (prt_lbm_sparse_*) are structure-of-arrays over the list:
+ *     f[i * ld + k], ld = prt_lbm_pop_stride(n_fluid, storage).
  */
 #ifndef PRTLB_H
 #define PRTLB_H
@@ -69,13 +73,17 @@ size_t prt_adr_params_size(void);
 
 /* ------------------------------------------------------------------ Stokes */
 
-/* Element stride between the population arrays of an n_nodes grid. */
+/* Elements of one population buffer of a dense (nx, ny, nz) grid (z-row
+ * layout above), or -1 for a bad grid. */
+int64_t prt_lbm_pop_elems(int q, int storage, int nx, int ny, int nz);
+
+/* Element stride between the population arrays of an n_nodes fluid list. */
 int64_t prt_lbm_pop_stride(int64_t n_nodes, int storage);
 
-/* Rest state f_i = w_i (zero deviation in the shifted storages) for n_nodes
- * nodes.  Replaces `f = np.ones(...) * w` (3D/tools/prtlb_3d.py:403,
- * 3D/tools/prtlb_2d.py:427). */
-int prt_lbm_fill_rest(int q, int storage, int64_t n_nodes, void *f,
+/* Rest state f_i = w_i (zero deviation in the shifted storages) over a
+ * whole dense population buffer.  Replaces `f = np.ones(...) * w`
+ * (3D/tools/prtlb_3d.py:403, 3D/tools/prtlb_2d.py:427). */
+int prt_lbm_fill_rest(int q, int storage, int nx, int ny, int nz, void *f,
                       void *stream);
 
 /* One fused TRT iteration: pull-stream from f_src, collide, Guo force,
@@ -268,7 +276,7 @@ int prt_adr_run(const prt_adr_params *p, int nx, int ny, int nz,
  * results alone or in any batch. */
 typedef struct prt_lbm_segment {
   const uint8_t *solid;   /* n bytes, device */
-  void *f_a, *f_b;        /* the segment's two population buffers (q * n) */
+  void *f_a, *f_b;        /* the segment's two population buffers (prt_lbm_pop_elems) */
   int32_t nx, ny, nz;     /* nz = 1 for D2Q9 */
   int32_t phase;          /* which buffer is newest at launch parity 0:
                              0 = f_a, 1 = f_b */

@@ -72,17 +72,18 @@ __global__ __launch_bounds__(256) void lbm_batch_kernel(const SegLbm *__restrict
     const bool solid = a.solid[n] != 0;
     if (!(SKIP && solid)) {
       const Nbr nb = make_nbr(x, y, z, a.X, a.Y, a.Z);
+      const size_t self = (size_t)nb.own(Q, (int)a.zs);
       double ux;
       if constexpr (ST == PRT_STORE_FP64) {
         double f[Q], fp[Q];
-        gather<Q, ST, SKIP>(a, nb, n, f);
+        gather<Q, ST, SKIP>(a, nb, f);
         collide_ref<Q>(a, solid, f, fp, ux);
-        scatter<Q, ST>(a, n, fp);
+        scatter<Q, ST>(a, self, a.zs, fp);
       } else {
         float f[Q], fp[Q];
-        gather<Q, ST, SKIP>(a, nb, n, f);
+        gather<Q, ST, SKIP>(a, nb, f);
         collide_dev<Q>(a, solid, f, fp, ux);
-        scatter<Q, ST>(a, n, fp);
+        scatter<Q, ST>(a, self, a.zs, fp);
       }
       if (RED && !solid) contrib = fabs(ux);
     }
@@ -115,12 +116,12 @@ __global__ __launch_bounds__(256) void lbm_batch_sparse_kernel(const SegLbm *__r
       double f[Q], fp[Q];
       gather_sparse<Q, ST>(a, d.link, d.lk, k, f);
       collide_ref<Q>(a, false, f, fp, ux);
-      scatter<Q, ST>(a, k, fp);
+      scatter<Q, ST>(a, (size_t)k, a.ld, fp);
     } else {
       float f[Q], fp[Q];
       gather_sparse<Q, ST>(a, d.link, d.lk, k, f);
       collide_dev<Q>(a, false, f, fp, ux);
-      scatter<Q, ST>(a, k, fp);
+      scatter<Q, ST>(a, (size_t)k, a.ld, fp);
     }
     if (RED) contrib = fabs(ux);
   }
@@ -220,8 +221,10 @@ extern "C" int prt_lbm_batch_plan(int q, int storage, const prt_lbm_segment *seg
       a.X = g.nx; a.Y = g.ny; a.Z = g.nz;
     }
     a.N = sparse ? (long long)g.n_fluid : (long long)g.nx * g.ny * g.nz;
-    a.ld = pop_stride(a.N, storage == PRT_STORE_FP64 ? 8 : (storage == PRT_STORE_FP32 ? 4 : 2));
-    if (a.ld * q >= (1LL << 31)) {
+    const int eb = storage == PRT_STORE_FP64 ? 8 : (storage == PRT_STORE_FP32 ? 4 : 2);
+    a.ld = sparse ? pop_stride(a.N, eb) : 0;
+    a.zs = sparse ? 0 : pop_row_pitch(a.Z, eb);
+    if ((sparse ? a.ld * q : pop_elems(q, a.X, a.Y, a.Z, eb)) >= (1LL << 31)) {
       set_error("prt_lbm_batch_plan: segment %d too large for the 32-bit index", s);
       return PRT_E_RANGE;
     }

@@ -79,8 +79,8 @@ extern "C" int prt_lbm_solver_create(int q, int storage, int nx, int ny, int nz,
 
 extern "C" int prt_lbm_solver_init(prt_lbm_solver *h) {
   if (!h) return PRT_E_ARG;
-  int rc = prt_lbm_fill_rest(h->q, h->storage, h->n, h->f[0], h->stream);
-  if (!rc) rc = prt_lbm_fill_rest(h->q, h->storage, h->n, h->f[1], h->stream);
+  int rc = prt_lbm_fill_rest(h->q, h->storage, h->nx, h->ny, h->nz, h->f[0], h->stream);
+  if (!rc) rc = prt_lbm_fill_rest(h->q, h->storage, h->nx, h->ny, h->nz, h->f[1], h->stream);
   h->latest = 0;
   return rc;
 }

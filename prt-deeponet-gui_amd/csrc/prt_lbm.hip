@@ -1,6 +1,8 @@
 // prt_lbm.hip -- fused TRT lattice-Boltzmann Stokes kernels for gfx950.
 //
-// One thread per node, pull scheme over a structure-of-arrays A/B pair:
+// One thread per node, pull scheme over an A/B pair of population buffers
+// stored by z-row (each row holds its Q populations one after the other,
+// each a contiguous run along z; prt_lbm_core.h):
 // every launch gathers the Q populations that stream INTO a node (periodic on
 // every axis), collides them (TRT with the magic parameter, first-order Guo
 // force with omega-), applies the solid-node reversal and writes the node's
@@ -51,16 +53,17 @@ __global__ __launch_bounds__(512) void lbm_step_kernel(LbmArgs a) {
       if (!(SKIP && solid)) {
         const Nbr nb = make_nbr(x, y, z, a.X, a.Y, a.Z);
         double ux;
+        const size_t self = (size_t)nb.own(Q, (int)a.zs);
         if constexpr (ST == PRT_STORE_FP64) {
           double f[Q], fp[Q];
-          gather<Q, ST, SKIP, double, NTL>(a, nb, n, f);
+          gather<Q, ST, SKIP, double, NTL>(a, nb, f);
           collide_ref<Q>(a, solid, f, fp, ux);
-          scatter<Q, ST>(a, n, fp);
+          scatter<Q, ST>(a, self, a.zs, fp);
         } else {
           float f[Q], fp[Q];
-          gather<Q, ST, SKIP, float, NTL>(a, nb, n, f);
+          gather<Q, ST, SKIP, float, NTL>(a, nb, f);
           collide_dev<Q>(a, solid, f, fp, ux);
-          scatter<Q, ST>(a, n, fp);
+          scatter<Q, ST>(a, self, a.zs, fp);
         }
         if (RED && !solid) contrib += fabs(ux);
       }
@@ -89,7 +92,7 @@ __global__ __launch_bounds__(256) void lbm_velocity_kernel(LbmArgs a, double *v6
     double u[3];
     if constexpr (ST == PRT_STORE_FP64) {
       double f[Q];
-      gather<Q, ST, SKIP>(a, nb, n, f);
+      gather<Q, ST, SKIP>(a, nb, f);
       double rho = f[0];
 #pragma unroll
       for (int i = 1; i < Q; ++i) rho += f[i];
@@ -105,7 +108,7 @@ __global__ __launch_bounds__(256) void lbm_velocity_kernel(LbmArgs a, double *v6
       u[2] = j[2] / rho;
     } else {
       float f[Q];
-      gather<Q, ST, SKIP>(a, nb, n, f);
+      gather<Q, ST, SKIP>(a, nb, f);
       double dr = 0.0, j[3] = {0.0, 0.0, 0.0};
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
@@ -141,13 +144,15 @@ __global__ __launch_bounds__(256) void lbm_velocity_kernel(LbmArgs a, double *v6
   }
 }
 
+// Rest state over a whole z-row buffer (row pitch zs, `total` elements),
+// run padding included.
 template <int Q, int ST>
-__global__ void fill_rest_kernel(void *f, long long ld) {
+__global__ void fill_rest_kernel(void *f, long long zs, long long total) {
   using T = typename Pop<ST>::T;
   T *p = static_cast<T *>(f);
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < ld * Q; k += stride) {
-    const int i = (int)(k / ld);
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
+    const int i = (int)((k / zs) % Q);
     if constexpr (ST == PRT_STORE_FP64)
       p[k] = weight<Q>(i);
     else
@@ -213,7 +218,9 @@ static int make_geom(int q, int nx, int ny, int nz, Geom &g, int flags = 0) {
     g.X = nx; g.Y = ny; g.Z = nz;
   }
   g.N = (long long)nx * ny * nz;
-  if (g.N * q >= (1LL << 31) || g.X > 65535) {
+  // every population element index (z-row layout, the widest row pitch of
+  // the three storages) stays below 2^31
+  if ((long long)g.X * g.Y * q * pop_row_pitch(g.Z, 2) >= (1LL << 31) || g.X > 65535) {
     set_error("grid of %lld nodes exceeds the 32-bit node index of this build", g.N);
     return PRT_E_RANGE;
   }
@@ -241,14 +248,23 @@ static int make_geom(int q, int nx, int ny, int nz, Geom &g, int flags = 0) {
   return PRT_OK;
 }
 
+long long pop_row_pitch(int Z, int elem_bytes) {
+  const long long line = 128 / elem_bytes;
+  return ((long long)Z + line - 1) / line * line;
+}
+
+long long pop_elems(int q, int X, int Y, int Z, int elem_bytes) {
+  return (long long)X * Y * q * pop_row_pitch(Z, elem_bytes);
+}
+
 long long pop_stride(long long n, int elem_bytes) {
   static long long env_pad = -2;
   if (env_pad == -2) {
     const char *e = std::getenv("PRTLB_POP_PAD_BYTES");
     env_pad = e ? std::atoll(e) : -1;
   }
-  // 4 KiB + 256 B: measured best of {0, 4352, 69888, 1048832} bytes on
-  // 128^3 and 256^3 D3Q19 in all three storages
+  // 4 KiB + 256 B: measured best of {0, 4352, 69888, 1048832} bytes for
+  // Q separate arrays over 128^3 and 256^3 D3Q19 grids in all three storages
   const long long pad_bytes = env_pad >= 0 ? env_pad : 4352;
   long long pad = (pad_bytes + elem_bytes - 1) / elem_bytes;
   // keep every array 256-byte aligned
@@ -264,7 +280,7 @@ static void fill_args(LbmArgs &a, int q, const Geom &g, const uint8_t *solid, co
   std::memset(&a, 0, sizeof(a));
   a.solid = solid; a.src = src; a.dst = dst; a.partials = partials;
   a.X = g.X; a.Y = g.Y; a.Z = g.Z; a.N = g.N;
-  a.ld = pop_stride(g.N, elem_bytes);
+  a.zs = pop_row_pitch(g.Z, elem_bytes);
   a.omp = omp; a.omm = omm; a.g = force; a.half_g = 0.5 * force;
   a.omh = 1 - 0.5 * omm;
   (void)q;
@@ -337,21 +353,25 @@ static int check_omega(double omp, double omm, double s16, int storage) {
 
 using namespace prt;
 
-extern "C" int prt_lbm_fill_rest(int q, int storage, int64_t n_nodes, void *f, void *stream) {
-  if (!f || n_nodes < 1 || (q != 9 && q != 19) || storage < 0 || storage > 2) {
+extern "C" int prt_lbm_fill_rest(int q, int storage, int nx, int ny, int nz, void *f, void *stream) {
+  Geom g;
+  if (!f || storage < 0 || storage > 2) {
     set_error("prt_lbm_fill_rest: bad arguments");
     return PRT_E_ARG;
   }
+  const int rc = make_geom(q, nx, ny, nz, g);
+  if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   const int blocks = 2048;
-  const long long ld = pop_stride(n_nodes, elem_size(storage));
+  const int eb = elem_size(storage);
+  const long long zs = pop_row_pitch(g.Z, eb), tot = pop_elems(q, g.X, g.Y, g.Z, eb);
   switch (q * 4 + storage) {
-    case 19 * 4 + PRT_STORE_FP64: fill_rest_kernel<19, PRT_STORE_FP64><<<blocks, 256, 0, st>>>(f, ld); break;
-    case 19 * 4 + PRT_STORE_FP32: fill_rest_kernel<19, PRT_STORE_FP32><<<blocks, 256, 0, st>>>(f, ld); break;
-    case 19 * 4 + PRT_STORE_FP16: fill_rest_kernel<19, PRT_STORE_FP16><<<blocks, 256, 0, st>>>(f, ld); break;
-    case 9 * 4 + PRT_STORE_FP64: fill_rest_kernel<9, PRT_STORE_FP64><<<blocks, 256, 0, st>>>(f, ld); break;
-    case 9 * 4 + PRT_STORE_FP32: fill_rest_kernel<9, PRT_STORE_FP32><<<blocks, 256, 0, st>>>(f, ld); break;
-    case 9 * 4 + PRT_STORE_FP16: fill_rest_kernel<9, PRT_STORE_FP16><<<blocks, 256, 0, st>>>(f, ld); break;
+    case 19 * 4 + PRT_STORE_FP64: fill_rest_kernel<19, PRT_STORE_FP64><<<blocks, 256, 0, st>>>(f, zs, tot); break;
+    case 19 * 4 + PRT_STORE_FP32: fill_rest_kernel<19, PRT_STORE_FP32><<<blocks, 256, 0, st>>>(f, zs, tot); break;
+    case 19 * 4 + PRT_STORE_FP16: fill_rest_kernel<19, PRT_STORE_FP16><<<blocks, 256, 0, st>>>(f, zs, tot); break;
+    case 9 * 4 + PRT_STORE_FP64: fill_rest_kernel<9, PRT_STORE_FP64><<<blocks, 256, 0, st>>>(f, zs, tot); break;
+    case 9 * 4 + PRT_STORE_FP32: fill_rest_kernel<9, PRT_STORE_FP32><<<blocks, 256, 0, st>>>(f, zs, tot); break;
+    case 9 * 4 + PRT_STORE_FP16: fill_rest_kernel<9, PRT_STORE_FP16><<<blocks, 256, 0, st>>>(f, zs, tot); break;
     default:
       set_error("unknown storage %d", storage);
       return PRT_E_ARG;
@@ -373,6 +393,16 @@ extern "C" int64_t prt_lbm_partials_count(int q, int nx, int ny, int nz) {
   if (make_geom(q, nx, ny, nz, g) != PRT_OK) return -1;
   const long long zb = (g.Z + 63) / 64;
   return (int64_t)(zb * g.Y * g.X);
+}
+
+extern "C" int64_t prt_lbm_pop_elems(int q, int storage, int nx, int ny, int nz) {
+  Geom g;
+  if (storage < 0 || storage > 2) {
+    set_error("prt_lbm_pop_elems: bad storage %d", storage);
+    return -1;
+  }
+  if (make_geom(q, nx, ny, nz, g) != PRT_OK) return -1;
+  return pop_elems(q, g.X, g.Y, g.Z, elem_size(storage));
 }
 
 extern "C" int64_t prt_lbm_pop_stride(int64_t n_nodes, int storage) {

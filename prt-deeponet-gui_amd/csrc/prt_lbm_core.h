@@ -48,18 +48,33 @@ struct LbmArgs {
   double *partials;
   int X, Y, Z;
   long long N;
-  long long ld;       // element stride between population arrays (>= N)
+  long long ld;       // fluid list: element stride between population arrays (>= N)
+  long long zs;       // dense grid: row pitch, elements per population per z-row (>= Z)
   double omp, omm, g, half_g;
   double omh;         // 1 - omm/2, formed on the host as the reference does
   float s16, inv_s16; // FP16 storage scale s and 1/s
 };
 
-// Element stride between the population arrays of an n-node grid.  The
-// arrays are padded so that the Q read streams and Q write streams of the
-// pull kernel start at staggered HBM channel offsets instead of the same
-// offset modulo a large power of two (the grids here are often 2^k cubed).
-// PRTLB_POP_PAD_BYTES overrides the pad (for tuning); the rule is the same
-// for every entry point, so buffers sized with prt_lbm_pop_stride() fit.
+// Dense grids store populations by z-row: the Q populations of row
+// (x, y) follow one another, each a contiguous run of zs elements,
+//
+//   element (i, x, y, z) at ((x*Y + y)*Q + i)*zs + z,
+//
+// with zs = Z rounded up to a whole number of 128-byte lines.  A block
+// that updates whole rows then writes one contiguous run of Q*zs elements
+// per row, and reads each population from a contiguous run of its source
+// row, instead of touching Q separate arrays across the whole buffer: far
+// fewer DRAM pages open at once.  Every population run starts on a cache
+// line, so no line is shared by two runs.
+long long pop_row_pitch(int Z, int elem_bytes);
+long long pop_elems(int q, int X, int Y, int Z, int elem_bytes);
+
+// Element stride between the population arrays of an n-entry fluid list.
+// The arrays are padded so that the Q read streams and Q write streams
+// start at staggered HBM channel offsets instead of the same offset modulo
+// a large power of two.  PRTLB_POP_PAD_BYTES overrides the pad (for
+// tuning); the rule is the same for every entry point, so buffers sized
+// with prt_lbm_pop_stride() fit.
 long long pop_stride(long long n, int elem_bytes);
 
 // Element stride of the link-table rows of an n_fluid fluid list.
@@ -111,6 +126,16 @@ struct Nbr {
     const int sz = ez > 0 ? zm : (ez < 0 ? zp : z);
     return (sx * Y + sy) * Z + sz;
   }
+  // element of population i of that node in the z-row layout (q, zs as
+  // in LbmArgs); below 2^31 by the grid check
+  __device__ inline int elem(int ex, int ey, int ez, int i, int q, int zs) const {
+    const int sx = ex > 0 ? xm : (ex < 0 ? xp : x);
+    const int sy = ey > 0 ? ym : (ey < 0 ? yp : y);
+    const int sz = ez > 0 ? zm : (ez < 0 ? zp : z);
+    return ((sx * Y + sy) * q + i) * zs + sz;
+  }
+  // element of population 0 of (x, y, z) itself
+  __device__ inline int own(int q, int zs) const { return (x * Y + y) * q * zs + z; }
 };
 
 __device__ inline Nbr make_nbr(int x, int y, int z, int X, int Y, int Z) {
@@ -125,23 +150,26 @@ __device__ inline Nbr make_nbr(int x, int y, int z, int X, int Y, int Z) {
   return b;
 }
 
-// Load the Q populations streaming into node n (SKIP: from the node's own
-// two-step-old opposite slot where the upstream node is solid).
+// Load the Q populations streaming into the node of nb (z-row layout;
+// SKIP: from the node's own two-step-old opposite slot where the upstream
+// node is solid).
 template <int Q, int ST, bool SKIP, typename R, bool NTL = false>
-__device__ inline void gather(const LbmArgs &a, const Nbr &nb, int n, R *f) {
+__device__ inline void gather(const LbmArgs &a, const Nbr &nb, R *f) {
   using T = typename Pop<ST>::T;
   const T *src = static_cast<const T *>(a.src);
   const T *own = static_cast<const T *>(a.dst);
+  const int zs = (int)a.zs;
+  const int self = nb.own(Q, zs);
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    const int s = (i == 0) ? n : nb.at(Lat<Q>::cx[i], Lat<Q>::cy[i], Lat<Q>::cz[i]);
+    const int e = (i == 0) ? self : nb.elem(Lat<Q>::cx[i], Lat<Q>::cy[i], Lat<Q>::cz[i], i, Q, zs);
     T v;
-    if (SKIP && i != 0 && a.solid[s])
-      v = own[(size_t)Lat<Q>::opp[i] * a.ld + n];
+    if (SKIP && i != 0 && a.solid[nb.at(Lat<Q>::cx[i], Lat<Q>::cy[i], Lat<Q>::cz[i])])
+      v = own[(size_t)self + (size_t)Lat<Q>::opp[i] * zs];
     else if constexpr (NTL && ST != PRT_STORE_FP16)
-      v = __builtin_nontemporal_load(src + (size_t)i * a.ld + s);   // read once: do not keep
+      v = __builtin_nontemporal_load(src + (size_t)e);   // read once: do not keep
     else
-      v = src[(size_t)i * a.ld + s];
+      v = src[(size_t)e];
     if constexpr (ST == PRT_STORE_FP16)
       f[i] = __half2float(v) * ld16<Q>(a, i);
     else
@@ -176,18 +204,21 @@ __device__ inline void gather_sparse(const LbmArgs &a, const int32_t *__restrict
   }
 }
 
+// Store the Q post-collision populations: population i at element
+// base + i * stride (dense: the node's own element and zs; fluid list:
+// the entry and ld).
 template <int Q, int ST, typename R>
-__device__ inline void scatter(const LbmArgs &a, int n, const R *fp) {
+__device__ inline void scatter(const LbmArgs &a, size_t base, long long stride, const R *fp) {
   using T = typename Pop<ST>::T;
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     if constexpr (ST == PRT_STORE_FP16) {
       unsigned short *dst = static_cast<unsigned short *>(a.dst);
       const unsigned short v = __half_as_ushort(__float2half_rn(fp[i] * st16<Q>(a, i)));
-      __builtin_nontemporal_store(v, dst + (size_t)i * a.ld + n);
+      __builtin_nontemporal_store(v, dst + base + (size_t)i * stride);
     } else {
       T *dst = static_cast<T *>(a.dst);
-      __builtin_nontemporal_store((T)fp[i], dst + (size_t)i * a.ld + n);
+      __builtin_nontemporal_store((T)fp[i], dst + base + (size_t)i * stride);
     }
   }
 }

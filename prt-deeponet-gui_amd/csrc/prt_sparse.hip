@@ -66,12 +66,12 @@ __global__ __launch_bounds__(kSparseBlock) void lbm_sparse_kernel(LbmArgs a, con
       double f[Q], fp[Q];
       gather_sparse<Q, ST>(a, link, lk, k, f);
       collide_ref<Q>(a, false, f, fp, ux);
-      scatter<Q, ST>(a, k, fp);
+      scatter<Q, ST>(a, (size_t)k, a.ld, fp);
     } else {
       float f[Q], fp[Q];
       gather_sparse<Q, ST>(a, link, lk, k, f);
       collide_dev<Q>(a, false, f, fp, ux);
-      scatter<Q, ST>(a, k, fp);
+      scatter<Q, ST>(a, (size_t)k, a.ld, fp);
     }
     if (RED) contrib = fabs(ux);
   }

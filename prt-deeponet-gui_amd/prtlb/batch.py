@@ -30,13 +30,17 @@ _STORE_BYTES = {"fp64": 8, "fp32": 4, "fp16": 2}
 
 
 def lbm_footprint(shape, storage, n_fluid=None):
-    """Device bytes one Stokes member holds: two population sets + mask over
-    the box, or, for a fluid list of n_fluid nodes, two compact population
-    sets, the link table, the list and the mask."""
+    """Device bytes one Stokes member holds: two population sets (z-rows
+    padded to whole 128-byte lines) + mask over the box, or, for a fluid
+    list of n_fluid nodes, two compact population sets, the link table, the
+    list and the mask."""
     n = int(np.prod(shape))
     q = 9 if len(shape) == 2 else 19
     if n_fluid is None:
-        return n * (2 * q * _STORE_BYTES[storage] + 1)
+        b = _STORE_BYTES[storage]
+        line = 128 // b
+        zs = (shape[-1] + line - 1) // line * line
+        return (n // shape[-1]) * zs * 2 * q * b + n
     return int(n_fluid) * (2 * q * _STORE_BYTES[storage] + 4 * (q - 1) + 4) + n
 
 
@@ -124,11 +128,11 @@ class LbmBatch:
             torch.cuda.current_stream(self.dev).synchronize()   # work is freed on return
         else:
             m.link, m.nodes = None, m.n
-            ld = int(lib().prt_lbm_pop_stride(m.n, self.st))
-            m.f = [torch.empty((self.q, ld), dtype=DTYPES[self.storage], device=self.dev)
+            elems = int(lib().prt_lbm_pop_elems(self.q, self.st, *m.dims))
+            m.f = [torch.empty(elems, dtype=DTYPES[self.storage], device=self.dev)
                    for _ in range(2)]
             for buf in m.f:
-                call("prt_lbm_fill_rest", self.q, self.st, m.n, ptr(buf), s)
+                call("prt_lbm_fill_rest", self.q, self.st, *m.dims, ptr(buf), s)
         m.phase = self.parity & 1            # newest = f[0] at the current parity
         m.iters = 0
         m.nit = int(nit)

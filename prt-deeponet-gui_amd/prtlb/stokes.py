@@ -116,8 +116,14 @@ class StokesSolver:
         self.flags = PRT_LBM_SKIP_SOLID if skip_solid else 0
         self.fp16_scale = float(fp16_scale)
         dt = DTYPES[storage]
-        self.ld = int(lib().prt_lbm_pop_stride(self.n, self.st))
-        self.f = [torch.empty((self.q, self.ld), dtype=dt, device=self.dev) for _ in range(2)]
+        elems = int(lib().prt_lbm_pop_elems(self.q, self.st, *self.dims))
+        if elems < 1:
+            raise PrtError("prt_lbm_pop_elems: " + lib().prt_last_error().decode(errors="replace"))
+        self.f = [torch.empty(elems, dtype=dt, device=self.dev) for _ in range(2)]
+        # z-row layout: rows (x, y) of the kernel's (X, Y, Z) grid, each
+        # holding q runs of zs elements (a 2D grid is (nx, 1, ny))
+        self.rows = (shape[0], 1, shape[1]) if self.nd == 2 else tuple(shape)
+        self.zs = elems // (self.rows[0] * self.rows[1] * self.q)
         self.tune = 0
         self.latest = 0          # index of the buffer holding the newest state
         self.iters = 0
@@ -126,7 +132,7 @@ class StokesSolver:
         self.scalar = torch.zeros(4, dtype=torch.float64, device=self.dev)
         s = self._stream()
         for buf in self.f:
-            call("prt_lbm_fill_rest", self.q, self.st, self.n, ptr(buf), s)
+            call("prt_lbm_fill_rest", self.q, self.st, *self.dims, ptr(buf), s)
 
     def _stream(self):
         return stream_ptr(self.dev)
@@ -234,8 +240,11 @@ class StokesSolver:
         return nu * (sum_ux / self.n) / self.force
 
     def populations(self):
-        """The newest post-collision populations (q, *grid) as stored."""
-        return self.f[self.latest][:, :self.n].reshape((self.q,) + self.shape)
+        """The newest post-collision populations as stored, gathered from
+        the z-row layout into a (q, *grid) tensor."""
+        X, Y, Z = self.rows
+        f = self.f[self.latest].view(X, Y, self.q, self.zs)[..., :Z]
+        return f.permute(2, 0, 1, 3).reshape((self.q,) + self.shape)
 
 
 def checked_flow(v, solid, tau_lb, force, name="flow"):

@@ -82,17 +82,16 @@ def test_fluid_list_step_equals_dense_fluid_only(q, storage):
     _lib.call("prt_lbm_sparse_run", q, st, nf, ptr(link), ptr(fa), ptr(fb), steps, S.om_p, S.om_m,
               force, S.fp16_scale, None, s)
     newest, other = (fb, fa) if steps % 2 else (fa, fb)
-    dense_new = S.f[S.latest] if hasattr(S, "latest") else None
+    dense_new = S.populations().reshape(q, -1)
     v_dense = S.velocity(f64=True)[1].cpu().numpy()
     v64 = torch.zeros((3, g.size), dtype=torch.float64, device=dev)
     part = torch.zeros(_lib.lib().prt_lbm_sparse_partials_count(nf), dtype=torch.float64, device=dev)
     _lib.call("prt_lbm_sparse_velocity", q, st, g.size, nf, ptr(fidx), ptr(link), ptr(newest),
               ptr(other), force, S.fp16_scale, ptr(v64), None, ptr(part), s)
     assert np.array_equal(v64.cpu().numpy().reshape(v_dense.shape), v_dense)
-    if dense_new is not None:
-        d = np.ascontiguousarray(dense_new[:, :g.size].cpu().numpy()[:, fidx.cpu().numpy()])
-        c = np.ascontiguousarray(newest[:, :nf].cpu().numpy())
-        assert np.array_equal(d.view(np.uint8), c.view(np.uint8))
+    d = np.ascontiguousarray(dense_new.cpu().numpy()[:, fidx.cpu().numpy()])
+    c = np.ascontiguousarray(newest[:, :nf].cpu().numpy())
+    assert np.array_equal(d.view(np.uint8), c.view(np.uint8))
 
 
 @pytest.mark.parametrize("storage", ["fp64", "fp32"])
