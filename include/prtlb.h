@@ -23,8 +23,10 @@
  *       f[((x * ny + y) * q + i) * zs + z],
  *     zs = nz rounded up to whole 128-byte lines (a 2D grid is the row set
  *     (nx, 1, ny)).  A buffer holds prt_lbm_pop_elems() elements.
- *   - fluid lists (prt_lbm_sparse_*) are structure-of-arrays over the list:
- *     f[i * ld + k], ld = prt_lbm_pop_stride(n_fluid, storage).
+ *   - fluid lists (prt_lbm_sparse_*) store populations by tile of 256
+ *     consecutive list entries, the q populations of a tile one after the
+ *     other: f[((k / 256) * q + i) * 256 + k % 256] for entry k.  A buffer
+ *     holds q * prt_lbm_pop_stride(n_fluid, storage) elements.
  */
 #ifndef PRTLB_H
 #define PRTLB_H
@@ -77,7 +79,8 @@ size_t prt_adr_params_size(void);
  * layout above), or -1 for a bad grid. */
 int64_t prt_lbm_pop_elems(int q, int storage, int nx, int ny, int nz);
 
-/* Element stride between the population arrays of an n_nodes fluid list. */
+/* Entries of an n_nodes fluid list rounded up to whole 256-entry tiles
+ * (a population buffer holds q times this many elements). */
 int64_t prt_lbm_pop_stride(int64_t n_nodes, int storage);
 
 /* Rest state f_i = w_i (zero deviation in the shifted storages) over a
@@ -145,8 +148,8 @@ int prt_plane_flux(const double *v64, int nx, int ny, int nz, double *plane,
 /* ------------------------------------------------ fluid-list (sparse) LBM */
 /* The same TRT step over the fluid nodes only, a sparse pore-node list (the
  * reference steps every box node, 3D/tools/prtlb_3d.py:403-445).  fluid_idx lists the n_fluid fluid (non-solid) box nodes in C
- * order; populations are f[i * ld + k] for list entry k with
- * ld = prt_lbm_pop_stride(n_fluid, storage); link[(i-1) * lk + k],
+ * order; populations are tiled by 256 entries (see the conventions at the
+ * top; q * prt_lbm_pop_stride(n_fluid, storage) elements); link[(i-1) * lk + k],
  * lk = prt_lbm_link_stride(n_fluid), is the entry that streams into entry k
  * along e_i (periodic) or -1 where that node is solid.  Results equal the
  * dense PRT_LBM_SKIP_SOLID ones bit for bit. */

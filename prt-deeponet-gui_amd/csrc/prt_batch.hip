@@ -116,12 +116,12 @@ __global__ __launch_bounds__(256) void lbm_batch_sparse_kernel(const SegLbm *__r
       double f[Q], fp[Q];
       gather_sparse<Q, ST>(a, d.link, d.lk, k, f);
       collide_ref<Q>(a, false, f, fp, ux);
-      scatter<Q, ST>(a, (size_t)k, a.ld, fp);
+      scatter<Q, ST>(a, (size_t)list_elem(k, 0, Q), kListTile, fp);
     } else {
       float f[Q], fp[Q];
       gather_sparse<Q, ST>(a, d.link, d.lk, k, f);
       collide_dev<Q>(a, false, f, fp, ux);
-      scatter<Q, ST>(a, (size_t)k, a.ld, fp);
+      scatter<Q, ST>(a, (size_t)list_elem(k, 0, Q), kListTile, fp);
     }
     if (RED) contrib = fabs(ux);
   }
@@ -222,9 +222,8 @@ extern "C" int prt_lbm_batch_plan(int q, int storage, const prt_lbm_segment *seg
     }
     a.N = sparse ? (long long)g.n_fluid : (long long)g.nx * g.ny * g.nz;
     const int eb = storage == PRT_STORE_FP64 ? 8 : (storage == PRT_STORE_FP32 ? 4 : 2);
-    a.ld = sparse ? pop_stride(a.N, eb) : 0;
     a.zs = sparse ? 0 : pop_row_pitch(a.Z, eb);
-    if ((sparse ? a.ld * q : pop_elems(q, a.X, a.Y, a.Z, eb)) >= (1LL << 31)) {
+    if ((sparse ? pop_stride(a.N) * q : pop_elems(q, a.X, a.Y, a.Z, eb)) >= (1LL << 31)) {
       set_error("prt_lbm_batch_plan: segment %d too large for the 32-bit index", s);
       return PRT_E_RANGE;
     }

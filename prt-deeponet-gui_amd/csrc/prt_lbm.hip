@@ -257,21 +257,7 @@ long long pop_elems(int q, int X, int Y, int Z, int elem_bytes) {
   return (long long)X * Y * q * pop_row_pitch(Z, elem_bytes);
 }
 
-long long pop_stride(long long n, int elem_bytes) {
-  static long long env_pad = -2;
-  if (env_pad == -2) {
-    const char *e = std::getenv("PRTLB_POP_PAD_BYTES");
-    env_pad = e ? std::atoll(e) : -1;
-  }
-  // 4 KiB + 256 B: measured best of {0, 4352, 69888, 1048832} bytes for
-  // Q separate arrays over 128^3 and 256^3 D3Q19 grids in all three storages
-  const long long pad_bytes = env_pad >= 0 ? env_pad : 4352;
-  long long pad = (pad_bytes + elem_bytes - 1) / elem_bytes;
-  // keep every array 256-byte aligned
-  long long ld = n + pad;
-  const long long al = 256 / elem_bytes;
-  return (ld + al - 1) / al * al;
-}
+long long pop_stride(long long n) { return (n + kListTile - 1) / kListTile * kListTile; }
 
 static int elem_size(int storage) { return storage == PRT_STORE_FP64 ? 8 : (storage == PRT_STORE_FP32 ? 4 : 2); }
 
@@ -410,7 +396,7 @@ extern "C" int64_t prt_lbm_pop_stride(int64_t n_nodes, int storage) {
     set_error("prt_lbm_pop_stride: bad arguments");
     return -1;
   }
-  return pop_stride(n_nodes, elem_size(storage));
+  return pop_stride(n_nodes);
 }
 
 extern "C" int prt_lbm_step(int q, int storage, int nx, int ny, int nz, const uint8_t *solid, const void *f_src,

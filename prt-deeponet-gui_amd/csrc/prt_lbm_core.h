@@ -48,7 +48,6 @@ struct LbmArgs {
   double *partials;
   int X, Y, Z;
   long long N;
-  long long ld;       // fluid list: element stride between population arrays (>= N)
   long long zs;       // dense grid: row pitch, elements per population per z-row (>= Z)
   double omp, omm, g, half_g;
   double omh;         // 1 - omm/2, formed on the host as the reference does
@@ -69,13 +68,20 @@ struct LbmArgs {
 long long pop_row_pitch(int Z, int elem_bytes);
 long long pop_elems(int q, int X, int Y, int Z, int elem_bytes);
 
-// Element stride between the population arrays of an n-entry fluid list.
-// The arrays are padded so that the Q read streams and Q write streams
-// start at staggered HBM channel offsets instead of the same offset modulo
-// a large power of two.  PRTLB_POP_PAD_BYTES overrides the pad (for
-// tuning); the rule is the same for every entry point, so buffers sized
-// with prt_lbm_pop_stride() fit.
-long long pop_stride(long long n, int elem_bytes);
+// Fluid lists store populations by tile of kListTile consecutive entries,
+// the same idea as the z-rows of a dense grid: the Q populations of a tile
+// follow one another, each a run of kListTile elements,
+//
+//   element (i, k) at ((k / kListTile)*Q + i)*kListTile + k % kListTile,
+//
+// so a 256-entry block writes one contiguous run of Q*kListTile elements.
+// A buffer holds Q * pop_stride(n) elements (n rounded up to a tile).
+constexpr int kListTile = 256;
+long long pop_stride(long long n);
+
+__device__ inline int list_elem(int k, int i, int q) {
+  return ((k / kListTile) * q + i) * kListTile + k % kListTile;
+}
 
 // Element stride of the link-table rows of an n_fluid fluid list.
 long long link_stride(long long n_fluid);
@@ -177,8 +183,8 @@ __device__ inline void gather(const LbmArgs &a, const Nbr &nb, R *f) {
   }
 }
 
-// Sparse (fluid-list) storage: only the fluid nodes' populations,
-// f[i * ld + k] for list entry k; link[(i - 1) * lk + k] is the list entry
+// Sparse (fluid-list) storage: only the fluid nodes' populations, tiled as
+// above (list_elem); link[(i - 1) * lk + k] is the list entry
 // of the node that streams into entry k along e_i, or -1 where that node is
 // solid -- then, as in gather<SKIP>, the entry's own two-step-old opposite
 // slot (still in the write buffer).  Same values as the dense fluid-only
@@ -195,7 +201,7 @@ __device__ inline void gather_sparse(const LbmArgs &a, const int32_t *__restrict
   for (int i = 1; i < Q; ++i) s[i] = link[(size_t)(i - 1) * lk + k];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    const T *p = (i == 0 || s[i] >= 0) ? src + (size_t)i * a.ld + s[i] : own + (size_t)Lat<Q>::opp[i] * a.ld + k;
+    const T *p = (i == 0 || s[i] >= 0) ? src + list_elem(s[i], i, Q) : own + list_elem(k, Lat<Q>::opp[i], Q);
     const T v = *p;
     if constexpr (ST == PRT_STORE_FP16)
       f[i] = __half2float(v) * ld16<Q>(a, i);
@@ -206,7 +212,7 @@ __device__ inline void gather_sparse(const LbmArgs &a, const int32_t *__restrict
 
 // Store the Q post-collision populations: population i at element
 // base + i * stride (dense: the node's own element and zs; fluid list:
-// the entry and ld).
+// list_elem(k, 0, Q) and kListTile).
 template <int Q, int ST, typename R>
 __device__ inline void scatter(const LbmArgs &a, size_t base, long long stride, const R *fp) {
   using T = typename Pop<ST>::T;

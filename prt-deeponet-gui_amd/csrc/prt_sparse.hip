@@ -66,12 +66,12 @@ __global__ __launch_bounds__(kSparseBlock) void lbm_sparse_kernel(LbmArgs a, con
       double f[Q], fp[Q];
       gather_sparse<Q, ST>(a, link, lk, k, f);
       collide_ref<Q>(a, false, f, fp, ux);
-      scatter<Q, ST>(a, (size_t)k, a.ld, fp);
+      scatter<Q, ST>(a, (size_t)list_elem(k, 0, Q), kListTile, fp);
     } else {
       float f[Q], fp[Q];
       gather_sparse<Q, ST>(a, link, lk, k, f);
       collide_dev<Q>(a, false, f, fp, ux);
-      scatter<Q, ST>(a, (size_t)k, a.ld, fp);
+      scatter<Q, ST>(a, (size_t)list_elem(k, 0, Q), kListTile, fp);
     }
     if (RED) contrib = fabs(ux);
   }
@@ -147,13 +147,14 @@ __global__ __launch_bounds__(kSparseBlock) void sparse_velocity_kernel(LbmArgs a
   }
 }
 
+// Rest state over a whole tiled list buffer of Q * ld elements.
 template <int Q, int ST>
 __global__ void sparse_fill_rest_kernel(void *f, long long ld) {
   using T = typename Pop<ST>::T;
   T *p = static_cast<T *>(f);
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < ld * Q; k += stride) {
-    const int i = (int)(k / ld);
+    const int i = (int)((k / kListTile) % Q);
     if constexpr (ST == PRT_STORE_FP64)
       p[k] = weight<Q>(i);
     else
@@ -161,11 +162,9 @@ __global__ void sparse_fill_rest_kernel(void *f, long long ld) {
   }
 }
 
-static int sp_elem(int storage) { return storage == PRT_STORE_FP64 ? 8 : (storage == PRT_STORE_FP32 ? 4 : 2); }
-
 static int sp_args(const char *who, int q, int storage, long long nf, const void *src, void *dst, double omp,
                    double omm, double force, double s16, LbmArgs &a) {
-  if ((q != 9 && q != 19) || storage < 0 || storage > 2 || nf < 1 || nf * q >= (1LL << 31) || !src || !dst ||
+  if ((q != 9 && q != 19) || storage < 0 || storage > 2 || nf < 1 || pop_stride(nf) * q >= (1LL << 31) || !src || !dst ||
       src == dst || !(omp > 0 && omp < 2) || !(omm > 0 && omm < 2) ||
       (storage == PRT_STORE_FP16 && !(s16 > 0))) {
     set_error("%s: bad arguments", who);
@@ -176,7 +175,6 @@ static int sp_args(const char *who, int q, int storage, long long nf, const void
   a.dst = dst;
   a.N = nf;
   a.X = a.Y = a.Z = 0;
-  a.ld = pop_stride(nf, sp_elem(storage));
   a.omp = omp;
   a.omm = omm;
   a.g = force;
@@ -223,7 +221,7 @@ extern "C" int prt_lbm_sparse_fill_rest(int q, int storage, int64_t n_fluid, voi
     set_error("prt_lbm_sparse_fill_rest: bad arguments");
     return PRT_E_ARG;
   }
-  const long long ld = pop_stride(n_fluid, sp_elem(storage));
+  const long long ld = pop_stride(n_fluid);
   hipStream_t st = (hipStream_t)stream;
   switch (q * 4 + storage) {
     case 19 * 4 + PRT_STORE_FP64: sparse_fill_rest_kernel<19, PRT_STORE_FP64><<<2048, 256, 0, st>>>(f, ld); break;

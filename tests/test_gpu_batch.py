@@ -90,7 +90,9 @@ def test_fluid_list_step_equals_dense_fluid_only(q, storage):
               ptr(other), force, S.fp16_scale, ptr(v64), None, ptr(part), s)
     assert np.array_equal(v64.cpu().numpy().reshape(v_dense.shape), v_dense)
     d = np.ascontiguousarray(dense_new.cpu().numpy()[:, fidx.cpu().numpy()])
-    c = np.ascontiguousarray(newest[:, :nf].cpu().numpy())
+    # tiled list layout: (tiles, q, 256) -> (q, entries)
+    c = newest.reshape(-1, q, 256).permute(1, 0, 2).reshape(q, -1)[:, :nf]
+    c = np.ascontiguousarray(c.cpu().numpy())
     assert np.array_equal(d.view(np.uint8), c.view(np.uint8))
 
 
