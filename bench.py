@@ -109,16 +109,17 @@ def time_fluid_list(g, storage, steps, warmup):
                 bytes_per_fluid_node=pop_b + link_b, gbs=gbs, frac=gbs * 1e9 / PEAK_HBM)
 
 
-def time_adr(g, steps, warmup, kinetics="monod", pore_list=True):
-    import numpy as np
+def time_adr(g, steps, warmup, kinetics="monod", pore_list=True, precision="fp32", v=None):
     import torch
     import prtlb
-    v = prtlb.stokes_d3q19(g, nit=1500, force=2e-6, tol=1e-5, storage="fp32")
+    if v is None:
+        v = prtlb.stokes_d3q19(g, nit=1500, force=2e-6, tol=1e-5, storage="fp32")
     if kinetics == "monod":
-        T = prtlb.TransportSolver(g, v, 10.0, 1.0, 1, chem=prtlb.chemistry.monod(0.5))
+        T = prtlb.TransportSolver(g, v, 10.0, 1.0, 1, chem=prtlb.chemistry.monod(0.5),
+                                  precision=precision)
     else:
         T = prtlb.TransportSolver(g, v, 10.0, 0.0, 2, chem=prtlb.chemistry.sorption_irreversible(),
-                                  da_abio=1.0)
+                                  da_abio=1.0, precision=precision)
     T.use_graph = False
     T.pore_list = pore_list
     T.step_n(warmup)
@@ -131,8 +132,8 @@ def time_adr(g, steps, warmup, kinetics="monod", pore_list=True):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
     vps = T.n_pore / (ms * 1e-3)
-    mobile = sum(int(T.params.mobile[j]) for j in range(T.ns))
-    bpv = 2 * T.ns * 4 + 3 * 4 + 1       # 21 B per pore voxel-step for ns = 1 (3D, f32)
+    cb = 4 if precision == "fp32" else 8
+    bpv = 2 * T.ns * cb + 3 * cb + 1     # 21 B per pore voxel-step for ns = 1 (3D, f32)
     return dict(ms_per_step=ms, pore_voxel_steps_per_s=vps, gbs=vps * bpv / 1e9,
                 frac=vps * bpv / PEAK_HBM, bytes_per_voxel=bpv, pore_voxels=T.n_pore)
 
@@ -399,10 +400,13 @@ def main():
         res = {}
         for E in (128, 256):
             g = prtlb.sphere_pack((E, E, E), (4.0, 8.0), 0.35, seed=0)
+            v = prtlb.stokes_d3q19(g, nit=1500, force=2e-6, tol=1e-5, storage="fp32")
             for kin in ("monod", "sorption"):
-                for pl in (True, False):
-                    res["adr_%d_%s_%s" % (E, kin, "list" if pl else "dense")] = time_adr(
-                        g, args.steps, args.warmup, kin, pl)
+                # pore list and dense box in float32, pore list in FP64
+                for name, pl, prec in (("list", True, "fp32"), ("dense", False, "fp32"),
+                                       ("list_f64", True, "fp64")):
+                    res["adr_%d_%s_%s" % (E, kin, name)] = time_adr(
+                        g, args.steps, args.warmup, kin, pl, prec, v=v)
         print(json.dumps({"adr": res}), flush=True)
         return
     if args.precision:
