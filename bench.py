@@ -83,14 +83,14 @@ def config1_pipeline():
                 transport_steps=ainfo["steps"], total_s=round(t2 - t0, 3))
 
 
-def time_fluid_list(g, storage, steps, warmup):
+def time_fluid_list(g, storage, steps, warmup, link16=True):
     """One geometry as a fluid-list member (compact populations + link
     table): time per step, fluid-node updates per second, and bandwidth
     counting the populations (2 Q s per fluid node) and the link table
-    (4 (Q - 1) bytes) it reads."""
+    (2 or 4 bytes per direction) it reads."""
     import torch
     from prtlb.batch import LbmBatch
-    b = LbmBatch(19, storage=storage)
+    b = LbmBatch(19, storage=storage, link16=link16)
     b.add(0, g, force=1e-6)
     b.step(warmup)
     torch.cuda.synchronize()
@@ -101,12 +101,14 @@ def time_fluid_list(g, storage, steps, warmup):
     e1.record(st)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
-    nf = b.members[0].n_fluid
+    m = b.members[0]
+    nf = m.n_fluid
     sb = {"fp64": 8, "fp32": 4, "fp16": 2}[storage]
-    pop_b, link_b = 2 * 19 * sb, 4 * 18
+    pop_b, link_b = 2 * 19 * sb, m.link_bits // 8 * 18
     gbs = nf * (pop_b + link_b) / (ms * 1e-3) / 1e9
     return dict(ms_per_step=ms, mlups_fluid=nf / (ms * 1e-3) / 1e6, fluid_nodes=nf,
-                bytes_per_fluid_node=pop_b + link_b, gbs=gbs, frac=gbs * 1e9 / PEAK_HBM)
+                link_bits=m.link_bits, bytes_per_fluid_node=pop_b + link_b, gbs=gbs,
+                frac=gbs * 1e9 / PEAK_HBM)
 
 
 def time_adr(g, steps, warmup, kinetics="monod", pore_list=True, precision="fp32", v=None):
@@ -443,6 +445,8 @@ def main():
             log("fluid list", st, "128^3 ...")
             extras["d3q19_128_%s_fluid_list" % st] = time_fluid_list(g128, st, args.steps,
                                                                      args.warmup)
+        extras["d3q19_128_fp32_fluid_list_link32"] = time_fluid_list(g128, "fp32", args.steps,
+                                                                     args.warmup, link16=False)
         log("geometry 256^3 ...")
         g256 = prtlb.sphere_pack((256, 256, 256), (4.0, 8.0), 0.35, seed=0)
         for st in ("fp64", "fp32", "fp16"):

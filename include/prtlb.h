@@ -163,18 +163,29 @@ int prt_lbm_sparse_fill_rest(int q, int storage, int64_t n_fluid, void *f,
 /* Partials length of the sparse run / velocity (one per 256 entries). */
 int prt_lbm_sparse_partials_count(int64_t n_fluid);
 /* As prt_lbm_run over the list (f_b must hold the state before f_a). */
-int prt_lbm_sparse_run(int q, int storage, int64_t n_fluid, const int32_t *link,
-                       void *f_a, void *f_b, int steps, double omega_p,
-                       double omega_m, double force, double fp16_scale,
-                       double *partials, void *stream);
+/* The link table as 16-bit steps: entry s of link[(i-1) * lk + k] becomes
+ * s - k taken modulo n_fluid into [-32767, 32767], and -32768 marks a solid
+ * upstream node.  Halves the table's bytes (and its share of every step's
+ * traffic: FP32 224 -> 188 B per fluid node).  link16 holds (q-1) * lk
+ * int16; *overflow (a device int32) becomes 1 if some step does not fit
+ * (a list with more than ~32k fluid nodes per x-plane): then keep using the
+ * 32-bit table. */
+int prt_lbm_link_pack16(int q, int64_t n_fluid, const int32_t *link,
+                        int16_t *link16, int32_t *overflow, void *stream);
+/* link_bits: 32 for the prt_lbm_link_table form, 16 for the packed one. */
+int prt_lbm_sparse_run(int q, int storage, int64_t n_fluid, const void *link,
+                       int link_bits, void *f_a, void *f_b, int steps,
+                       double omega_p, double omega_m, double force,
+                       double fp16_scale, double *partials, void *stream);
 /* As prt_lbm_velocity (fluid-only) over the list: writes the fluid entries
  * of v64 / v32 (3, n_box) at their box nodes; the caller zeroes the rest.
  * partials[b]: FP64 sum of u_x per block. */
 int prt_lbm_sparse_velocity(int q, int storage, int64_t n_box, int64_t n_fluid,
-                            const int32_t *fluid_idx, const int32_t *link,
-                            const void *f_last, const void *f_prev,
-                            double force, double fp16_scale, double *v64,
-                            float *v32, double *partials, void *stream);
+                            const int32_t *fluid_idx, const void *link,
+                            int link_bits, const void *f_last,
+                            const void *f_prev, double force,
+                            double fp16_scale, double *v64, float *v32,
+                            double *partials, void *stream);
 
 /* --------------------------------------------------------------- transport */
 
@@ -284,11 +295,14 @@ typedef struct prt_lbm_segment {
   int32_t phase;          /* which buffer is newest at launch parity 0:
                              0 = f_a, 1 = f_b */
   double omega_p, omega_m, force, fp16_scale;
-  const int32_t *link;    /* fluid-list member (PRT_LBM_SPARSE batches): its
-                             link table (prt_lbm_link_table); f_a / f_b then
+  const void *link;       /* fluid-list member (PRT_LBM_SPARSE batches): its
+                             link table (prt_lbm_link_table, or its 16-bit
+                             form from prt_lbm_link_pack16); f_a / f_b then
                              hold n_fluid entries per population; NULL for a
                              dense member.  All members of a batch agree. */
   int64_t n_fluid;
+  int32_t link_bits;      /* 32 (or 0) or 16: width of the link table */
+  int32_t reserved;
 } prt_lbm_segment;
 
 typedef struct prt_adr_segment {

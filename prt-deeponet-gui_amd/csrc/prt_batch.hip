@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <type_traits>
 
 #include "prt_adr_core.h"
 #include "prt_lbm_core.h"
@@ -28,9 +29,9 @@ namespace prt {
 struct SegLbm {
   LbmArgs a;        // src/dst chosen per launch from fa/fb and the phase
   void *fa, *fb;
-  const int32_t *link;  // fluid-list member: link table (a.N = list length)
+  const void *link;     // fluid-list member: link table (a.N = list length)
   long long lk;
-  int phase, blk0, nblk, pad;
+  int phase, blk0, nblk, link16;  // link16: 16-bit link table
 };
 
 struct SegAdr {
@@ -112,17 +113,13 @@ __global__ __launch_bounds__(256) void lbm_batch_sparse_kernel(const SegLbm *__r
   double contrib = 0.0;
   if (k < a.N) {
     double ux;
-    if constexpr (ST == PRT_STORE_FP64) {
-      double f[Q], fp[Q];
-      gather_sparse<Q, ST>(a, d.link, d.lk, k, f);
-      collide_ref<Q>(a, false, f, fp, ux);
-      scatter<Q, ST>(a, (size_t)list_elem(k, 0, Q), kListTile, fp);
-    } else {
-      float f[Q], fp[Q];
-      gather_sparse<Q, ST>(a, d.link, d.lk, k, f);
-      collide_dev<Q>(a, false, f, fp, ux);
-      scatter<Q, ST>(a, (size_t)list_elem(k, 0, Q), kListTile, fp);
-    }
+    using R = typename std::conditional<ST == PRT_STORE_FP64, double, float>::type;
+    R f[Q], fp[Q];
+    if (d.link16) gather_sparse<Q, ST>(a, static_cast<const int16_t *>(d.link), d.lk, k, f);
+    else gather_sparse<Q, ST>(a, static_cast<const int32_t *>(d.link), d.lk, k, f);
+    if constexpr (ST == PRT_STORE_FP64) collide_ref<Q>(a, false, f, fp, ux);
+    else collide_dev<Q>(a, false, f, fp, ux);
+    scatter<Q, ST>(a, (size_t)list_elem(k, 0, Q), kListTile, fp);
     if (RED) contrib = fabs(ux);
   }
   if (RED) {
@@ -203,7 +200,8 @@ extern "C" int prt_lbm_batch_plan(int q, int storage, const prt_lbm_segment *seg
     const bool sparse = g.link != nullptr;
     if (g.nx < 1 || g.ny < 1 || g.nz < 1 || (q == 9 && g.nz != 1) || (!sparse && !g.solid) || !g.f_a ||
         !g.f_b || g.f_a == g.f_b || (sparse && (g.n_fluid < 1 || g.n_fluid > (long long)g.nx * g.ny * g.nz)) ||
-        (s > 0 && sparse != (segs[0].link != nullptr))) {
+        (s > 0 && sparse != (segs[0].link != nullptr)) ||
+        (sparse && g.link_bits != 0 && g.link_bits != 16 && g.link_bits != 32)) {
       set_error("prt_lbm_batch_plan: segment %d has a bad grid or buffers", s);
       return PRT_E_ARG;
     }
@@ -235,6 +233,7 @@ extern "C" int prt_lbm_batch_plan(int q, int storage, const prt_lbm_segment *seg
     d[s].fa = g.f_a;
     d[s].fb = g.f_b;
     d[s].link = g.link;
+    d[s].link16 = sparse && g.link_bits == 16;
     d[s].lk = sparse ? link_stride(a.N) : 0;
     d[s].phase = g.phase & 1;
     d[s].blk0 = (int)total;

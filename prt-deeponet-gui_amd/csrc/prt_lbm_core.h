@@ -189,16 +189,32 @@ __device__ inline void gather(const LbmArgs &a, const Nbr &nb, R *f) {
 // solid -- then, as in gather<SKIP>, the entry's own two-step-old opposite
 // slot (still in the write buffer).  Same values as the dense fluid-only
 // gather, so the same bits after the collision.
-template <int Q, int ST, typename R>
-__device__ inline void gather_sparse(const LbmArgs &a, const int32_t *__restrict__ link, long long lk, int k,
-                                     R *f) {
+//
+// Link tables come in two widths.  int32: the entry itself, -1 for solid.
+// int16 (kLink16Solid for solid): the entry minus k taken modulo the list
+// length into [-32767, 32767], which every member fits whose fluid nodes
+// per x-plane stay below 32k (the periodic wrap along x is a small step
+// modulo the list); it halves the table's traffic, 4 (Q - 1) -> 2 (Q - 1)
+// bytes per fluid node.
+constexpr int kLink16Solid = -32768;
+
+__device__ inline int link_entry(const int32_t *__restrict__ link, size_t at, int, int) { return link[at]; }
+__device__ inline int link_entry(const int16_t *__restrict__ link, size_t at, int k, int nf) {
+  const int d = link[at];
+  if (d == kLink16Solid) return -1;
+  const int s = k + d;
+  return s < 0 ? s + nf : (s >= nf ? s - nf : s);
+}
+
+template <int Q, int ST, typename R, typename L>
+__device__ inline void gather_sparse(const LbmArgs &a, const L *__restrict__ link, long long lk, int k, R *f) {
   using T = typename Pop<ST>::T;
   const T *src = static_cast<const T *>(a.src);
   const T *own = static_cast<const T *>(a.dst);
   int s[Q];
   s[0] = k;
 #pragma unroll
-  for (int i = 1; i < Q; ++i) s[i] = link[(size_t)(i - 1) * lk + k];
+  for (int i = 1; i < Q; ++i) s[i] = link_entry(link, (size_t)(i - 1) * lk + k, k, (int)a.N);
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const T *p = (i == 0 || s[i] >= 0) ? src + list_elem(s[i], i, Q) : own + list_elem(k, Lat<Q>::opp[i], Q);

@@ -6,8 +6,9 @@
 // compacted in C order (entry k = the k-th fluid node), with a link table
 // giving, per direction, the entry of the node that streams into each entry
 // (periodic wrap as the dense grid) or -1 where that node is solid.  A step
-// then moves 2 Q s + 4 (Q - 1) bytes per FLUID node (FP32: 224 B), against
-// 2 Q s per BOX node dense (FP32: 152 B, i.e. 152 / porosity per fluid node).
+// then moves 2 Q s + 4 (Q - 1) bytes per FLUID node (FP32: 224 B; 188 B
+// with the 16-bit link table), against 2 Q s per BOX node dense (FP32:
+// 152 B, i.e. 152 / porosity per fluid node).
 //
 // The values are the dense fluid-only kernel's, bit for bit: the same gather
 // rule (a solid upstream node hands back the entry's own two-step-old
@@ -53,8 +54,32 @@ __global__ __launch_bounds__(256) void link_build_kernel(const int32_t *fluid_id
   for (int i = 1; i < Q; ++i) link[(size_t)(i - 1) * lk + k] = map[nb.at(Lat<Q>::cx[i], Lat<Q>::cy[i], Lat<Q>::cz[i])];
 }
 
-template <int Q, int ST, bool RED>
-__global__ __launch_bounds__(kSparseBlock) void lbm_sparse_kernel(LbmArgs a, const int32_t *__restrict__ link,
+// link[(i - 1) * lk + k] as a 16-bit step modulo nf (prt_lbm_core.h);
+// *overflow = 1 if some link does not fit.
+__global__ __launch_bounds__(256) void link_pack16_kernel(const int32_t *__restrict__ link, long long nf, long long lk,
+                                                          int rows, int16_t *__restrict__ out, int32_t *overflow) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nf) return;
+  bool bad = false;
+  for (int r = 0; r < rows; ++r) {
+    const int32_t v = link[(size_t)r * lk + k];
+    int16_t e = (int16_t)kLink16Solid;
+    if (v >= 0) {
+      long long d = (long long)v - k;
+      if (d > nf / 2) d -= nf;
+      if (d < -(nf / 2)) d += nf;
+      if (d < -32767 || d > 32767 || v >= nf) bad = true;
+      else e = (int16_t)d;
+    }
+    out[(size_t)r * lk + k] = e;
+  }
+  if (bad) atomicOr(overflow, 1);
+}
+
+__global__ void flag_clear_kernel(int32_t *p) { *p = 0; }
+
+template <int Q, int ST, bool RED, typename L>
+__global__ __launch_bounds__(kSparseBlock) void lbm_sparse_kernel(LbmArgs a, const L *__restrict__ link,
                                                                  long long lk) {
   __shared__ double lds[16];
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -84,8 +109,8 @@ __global__ __launch_bounds__(kSparseBlock) void lbm_sparse_kernel(LbmArgs a, con
 // Velocity of every fluid entry written at its box node (the caller zeroes
 // the solid nodes), moments as the dense velocity kernel; partials[b] = the
 // block's sum of u_x (FP64).
-template <int Q, int ST>
-__global__ __launch_bounds__(kSparseBlock) void sparse_velocity_kernel(LbmArgs a, const int32_t *__restrict__ link,
+template <int Q, int ST, typename L>
+__global__ __launch_bounds__(kSparseBlock) void sparse_velocity_kernel(LbmArgs a, const L *__restrict__ link,
                                                                       long long lk, const int32_t *fluid_idx,
                                                                       long long nbox, double *v64, float *v32) {
 #pragma clang fp contract(off)
@@ -238,16 +263,31 @@ extern "C" int prt_lbm_sparse_partials_count(int64_t n_fluid) {
   return (int)((n_fluid + kSparseBlock - 1) / kSparseBlock);
 }
 
-extern "C" int prt_lbm_sparse_run(int q, int storage, int64_t n_fluid, const int32_t *link, void *f_a, void *f_b,
-                                  int steps, double omega_p, double omega_m, double force, double fp16_scale,
-                                  double *partials, void *stream) {
+extern "C" int prt_lbm_link_pack16(int q, int64_t n_fluid, const int32_t *link, int16_t *link16, int32_t *overflow,
+                                   void *stream) {
+  if ((q != 9 && q != 19) || n_fluid < 1 || !link || !link16 || !overflow) {
+    set_error("prt_lbm_link_pack16: bad arguments");
+    return PRT_E_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  flag_clear_kernel<<<1, 1, 0, st>>>(overflow);
+  link_pack16_kernel<<<(unsigned)((n_fluid + 255) / 256), 256, 0, st>>>(link, n_fluid, link_stride(n_fluid), q - 1,
+                                                                        link16, overflow);
+  return launch_status("prt_lbm_link_pack16 launch");
+}
+
+extern "C" int prt_lbm_sparse_run(int q, int storage, int64_t n_fluid, const void *link, int link_bits, void *f_a,
+                                  void *f_b, int steps, double omega_p, double omega_m, double force,
+                                  double fp16_scale, double *partials, void *stream) {
   LbmArgs a;
   int rc = sp_args("prt_lbm_sparse_run", q, storage, n_fluid, f_a, f_b, omega_p, omega_m, force, fp16_scale, a);
   if (rc) return rc;
-  if (!link || steps < 0) {
-    set_error("prt_lbm_sparse_run: null link table or negative steps");
+  if (!link || steps < 0 || (link_bits != 32 && link_bits != 16)) {
+    set_error("prt_lbm_sparse_run: null link table, link width not 16 or 32, or negative steps");
     return PRT_E_ARG;
   }
+  const int32_t *l32 = static_cast<const int32_t *>(link);
+  const int16_t *l16 = static_cast<const int16_t *>(link);
   const long long lk = link_stride(n_fluid);
   const dim3 grid((unsigned)((n_fluid + kSparseBlock - 1) / kSparseBlock)), block(kSparseBlock);
   hipStream_t st = (hipStream_t)stream;
@@ -256,10 +296,15 @@ extern "C" int prt_lbm_sparse_run(int q, int storage, int64_t n_fluid, const int
     a.dst = (s & 1) ? f_a : f_b;
     const bool red = partials && s == steps - 1;
     a.partials = red ? partials : nullptr;
-#define PRT_SK(QQ, SS)                                                           \
-  do {                                                                           \
-    if (red) lbm_sparse_kernel<QQ, SS, true><<<grid, block, 0, st>>>(a, link, lk);  \
-    else lbm_sparse_kernel<QQ, SS, false><<<grid, block, 0, st>>>(a, link, lk);     \
+#define PRT_SK(QQ, SS)                                                                            \
+  do {                                                                                            \
+    if (link_bits == 16) {                                                                        \
+      if (red) lbm_sparse_kernel<QQ, SS, true><<<grid, block, 0, st>>>(a, l16, lk);               \
+      else lbm_sparse_kernel<QQ, SS, false><<<grid, block, 0, st>>>(a, l16, lk);                  \
+    } else {                                                                                      \
+      if (red) lbm_sparse_kernel<QQ, SS, true><<<grid, block, 0, st>>>(a, l32, lk);               \
+      else lbm_sparse_kernel<QQ, SS, false><<<grid, block, 0, st>>>(a, l32, lk);                  \
+    }                                                                                             \
   } while (0)
     switch (q * 4 + storage) {
       case 19 * 4 + PRT_STORE_FP64: PRT_SK(19, PRT_STORE_FP64); break;
@@ -277,21 +322,30 @@ extern "C" int prt_lbm_sparse_run(int q, int storage, int64_t n_fluid, const int
 }
 
 extern "C" int prt_lbm_sparse_velocity(int q, int storage, int64_t n_box, int64_t n_fluid, const int32_t *fluid_idx,
-                                       const int32_t *link, const void *f_last, const void *f_prev, double force,
-                                       double fp16_scale, double *v64, float *v32, double *partials, void *stream) {
+                                       const void *link, int link_bits, const void *f_last, const void *f_prev,
+                                       double force, double fp16_scale, double *v64, float *v32, double *partials,
+                                       void *stream) {
   LbmArgs a;
   int rc = sp_args("prt_lbm_sparse_velocity", q, storage, n_fluid, f_last, const_cast<void *>(f_prev), 1.0, 1.0,
                    force, fp16_scale, a);
   if (rc) return rc;
-  if (!fluid_idx || !link || n_box < n_fluid || (!v64 && !v32)) {
+  if (!fluid_idx || !link || n_box < n_fluid || (!v64 && !v32) || (link_bits != 32 && link_bits != 16)) {
     set_error("prt_lbm_sparse_velocity: bad arguments");
     return PRT_E_ARG;
   }
+  const int32_t *l32 = static_cast<const int32_t *>(link);
+  const int16_t *l16 = static_cast<const int16_t *>(link);
   a.partials = partials;
   const long long lk = link_stride(n_fluid);
   const dim3 grid((unsigned)((n_fluid + kSparseBlock - 1) / kSparseBlock)), block(kSparseBlock);
   hipStream_t st = (hipStream_t)stream;
-#define PRT_SV(QQ, SS) sparse_velocity_kernel<QQ, SS><<<grid, block, 0, st>>>(a, link, lk, fluid_idx, n_box, v64, v32)
+#define PRT_SV(QQ, SS)                                                                                      \
+  do {                                                                                                      \
+    if (link_bits == 16)                                                                                    \
+      sparse_velocity_kernel<QQ, SS><<<grid, block, 0, st>>>(a, l16, lk, fluid_idx, n_box, v64, v32);       \
+    else                                                                                                    \
+      sparse_velocity_kernel<QQ, SS><<<grid, block, 0, st>>>(a, l32, lk, fluid_idx, n_box, v64, v32);       \
+  } while (0)
   switch (q * 4 + storage) {
     case 19 * 4 + PRT_STORE_FP64: PRT_SV(19, PRT_STORE_FP64); break;
     case 19 * 4 + PRT_STORE_FP32: PRT_SV(19, PRT_STORE_FP32); break;

@@ -61,7 +61,8 @@ class LbmSegment(ctypes.Structure):
                 ("phase", ctypes.c_int32), ("omega_p", ctypes.c_double),
                 ("omega_m", ctypes.c_double), ("force", ctypes.c_double),
                 ("fp16_scale", ctypes.c_double), ("link", ctypes.c_void_p),
-                ("n_fluid", ctypes.c_int64)]
+                ("n_fluid", ctypes.c_int64), ("link_bits", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class AdrSegment(ctypes.Structure):
@@ -112,8 +113,10 @@ _SIGS = {
     "prt_lbm_link_table": (_I, [_I, _I, _I, _I, _P, _I64, _P, _P, _P]),
     "prt_lbm_sparse_fill_rest": (_I, [_I, _I, _I64, _P, _P]),
     "prt_lbm_sparse_partials_count": (_I, [_I64]),
-    "prt_lbm_sparse_run": (_I, [_I, _I, _I64, _P, _P, _P, _I, _D, _D, _D, _D, _P, _P]),
-    "prt_lbm_sparse_velocity": (_I, [_I, _I, _I64, _I64, _P, _P, _P, _P, _D, _D, _P, _P, _P, _P]),
+    "prt_lbm_link_pack16": (_I, [_I, _I64, _P, _P, _P, _P]),
+    "prt_lbm_sparse_run": (_I, [_I, _I, _I64, _P, _I, _P, _P, _I, _D, _D, _D, _D, _P, _P]),
+    "prt_lbm_sparse_velocity": (_I, [_I, _I, _I64, _I64, _P, _P, _I, _P, _P, _D, _D, _P, _P, _P,
+                                     _P]),
     "prt_geodesic": (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _I, _P, _P, _P]),
     "prt_edt": (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P]),
     "prt_check_blocks": (_I, [_I64]),

@@ -32,8 +32,8 @@ _STORE_BYTES = {"fp64": 8, "fp32": 4, "fp16": 2}
 def lbm_footprint(shape, storage, n_fluid=None):
     """Device bytes one Stokes member holds: two population sets (z-rows
     padded to whole 128-byte lines) + mask over the box, or, for a fluid
-    list of n_fluid nodes, two compact population sets, the link table, the
-    list and the mask."""
+    list of n_fluid nodes, two compact population sets, the link table (its
+    32-bit form, built first), the list and the mask."""
     n = int(np.prod(shape))
     q = 9 if len(shape) == 2 else 19
     if n_fluid is None:
@@ -67,7 +67,7 @@ class LbmBatch:
     """
 
     def __init__(self, q, storage="fp32", skip_solid=True, every=50, device=None,
-                 fp16_scale=1024.0, sparse=None):
+                 fp16_scale=1024.0, sparse=None, link16=True):
         if q not in (9, 19):
             raise ValueError("q must be 9 or 19")
         self.q, self.storage, self.st = q, storage, PRT_STORE[storage]
@@ -77,6 +77,8 @@ class LbmBatch:
         if self.sparse and not skip_solid:
             raise ValueError("a fluid-list batch is fluid-only (skip_solid=True)")
         self.flags = (PRT_LBM_SKIP_SOLID if skip_solid else 0) | (PRT_LBM_SPARSE if self.sparse else 0)
+        # fluid lists keep their link table as 16-bit steps where it fits
+        self.link16 = bool(link16)
         self.every = int(every)
         self.dev = _device(device)
         self.fp16_scale = float(fp16_scale)
@@ -119,6 +121,14 @@ class LbmBatch:
             work = torch.empty(m.n, dtype=torch.int32, device=self.dev)
             call("prt_lbm_link_table", self.q, *m.dims, ptr(m.fidx), m.n_fluid, ptr(work),
                  ptr(m.link), s)
+            m.link_bits = 32
+            if self.link16:
+                # the 16-bit form where every link fits (halves the table)
+                l16 = torch.empty((self.q - 1) * lk, dtype=torch.int16, device=self.dev)
+                over = torch.empty(1, dtype=torch.int32, device=self.dev)
+                call("prt_lbm_link_pack16", self.q, m.n_fluid, ptr(m.link), ptr(l16), ptr(over), s)
+                if int(over.item()) == 0:
+                    m.link, m.link_bits = l16, 16
             m.nodes = m.n_fluid
             ld = int(lib().prt_lbm_pop_stride(m.n_fluid, self.st))
             m.f = [torch.empty((self.q, ld), dtype=DTYPES[self.storage], device=self.dev)
@@ -158,7 +168,8 @@ class LbmBatch:
         for i, m in enumerate(mem):
             segs[i] = LbmSegment(ptr(m.solid), ptr(m.f[0]), ptr(m.f[1]), *m.dims, m.phase,
                                  m.om_p, m.om_m, m.force, self.fp16_scale,
-                                 ptr(m.link) if m.link is not None else None, m.nodes)
+                                 ptr(m.link) if m.link is not None else None, m.nodes,
+                                 m.link_bits if m.link is not None else 0, 0)
         dsz = lib().prt_lbm_batch_desc_size()
         desc = np.zeros(dsz * len(mem), np.uint8)
         nb = ctypes.c_int64()
@@ -255,8 +266,8 @@ class LbmBatch:
             part = torch.zeros(lib().prt_lbm_sparse_partials_count(m.n_fluid), dtype=torch.float64,
                                device=self.dev)
             call("prt_lbm_sparse_velocity", self.q, self.st, m.n, m.n_fluid, ptr(m.fidx),
-                 ptr(m.link), ptr(newest), ptr(other), m.force, self.fp16_scale, None, ptr(v32),
-                 ptr(part), s)
+                 ptr(m.link), m.link_bits, ptr(newest), ptr(other), m.force, self.fp16_scale, None,
+                 ptr(v32), ptr(part), s)
         else:
             v32 = torch.empty((3, m.n), dtype=torch.float32, device=self.dev)
             part = torch.zeros(lib().prt_lbm_partials_count(self.q, *m.dims), dtype=torch.float64,

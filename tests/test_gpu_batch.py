@@ -79,14 +79,14 @@ def test_fluid_list_step_equals_dense_fluid_only(q, storage):
     fb = torch.empty_like(fa)
     for b in (fa, fb):
         _lib.call("prt_lbm_sparse_fill_rest", q, st, nf, ptr(b), s)
-    _lib.call("prt_lbm_sparse_run", q, st, nf, ptr(link), ptr(fa), ptr(fb), steps, S.om_p, S.om_m,
+    _lib.call("prt_lbm_sparse_run", q, st, nf, ptr(link), 32, ptr(fa), ptr(fb), steps, S.om_p, S.om_m,
               force, S.fp16_scale, None, s)
     newest, other = (fb, fa) if steps % 2 else (fa, fb)
     dense_new = S.populations().reshape(q, -1)
     v_dense = S.velocity(f64=True)[1].cpu().numpy()
     v64 = torch.zeros((3, g.size), dtype=torch.float64, device=dev)
     part = torch.zeros(_lib.lib().prt_lbm_sparse_partials_count(nf), dtype=torch.float64, device=dev)
-    _lib.call("prt_lbm_sparse_velocity", q, st, g.size, nf, ptr(fidx), ptr(link), ptr(newest),
+    _lib.call("prt_lbm_sparse_velocity", q, st, g.size, nf, ptr(fidx), ptr(link), 32, ptr(newest),
               ptr(other), force, S.fp16_scale, ptr(v64), None, ptr(part), s)
     assert np.array_equal(v64.cpu().numpy().reshape(v_dense.shape), v_dense)
     d = np.ascontiguousarray(dense_new.cpu().numpy()[:, fidx.cpu().numpy()])
@@ -94,6 +94,56 @@ def test_fluid_list_step_equals_dense_fluid_only(q, storage):
     c = newest.reshape(-1, q, 256).permute(1, 0, 2).reshape(q, -1)[:, :nf]
     c = np.ascontiguousarray(c.cpu().numpy())
     assert np.array_equal(d.view(np.uint8), c.view(np.uint8))
+
+    # the 16-bit link table: decodes to the 32-bit one and gives the same bits
+    l16 = torch.empty((q - 1) * lk, dtype=torch.int16, device=dev)
+    over = torch.ones(1, dtype=torch.int32, device=dev)
+    _lib.call("prt_lbm_link_pack16", q, nf, ptr(link), ptr(l16), ptr(over), s)
+    assert int(over.item()) == 0
+    d16 = l16.cpu().numpy().reshape(q - 1, lk)[:, :nf].astype(np.int64)
+    k = np.arange(nf)
+    dec = np.where(d16 == -32768, -1, (k + d16) % nf)
+    assert np.array_equal(dec, link.cpu().numpy().reshape(q - 1, lk)[:, :nf])
+    ga, gb = torch.empty_like(fa), torch.empty_like(fa)
+    for b in (ga, gb):
+        _lib.call("prt_lbm_sparse_fill_rest", q, st, nf, ptr(b), s)
+    _lib.call("prt_lbm_sparse_run", q, st, nf, ptr(l16), 16, ptr(ga), ptr(gb), steps, S.om_p,
+              S.om_m, force, S.fp16_scale, None, s)
+    new16, other16 = (gb, ga) if steps % 2 else (ga, gb)
+    assert torch.equal(new16.view(torch.uint8), newest.view(torch.uint8))
+    v16 = torch.zeros_like(v64)
+    _lib.call("prt_lbm_sparse_velocity", q, st, g.size, nf, ptr(fidx), ptr(l16), 16, ptr(new16),
+              ptr(other16), force, S.fp16_scale, ptr(v16), None, ptr(part), s)
+    assert torch.equal(v16, v64)
+
+
+def test_link_pack16_reports_overflow():
+    """A list with more fluid nodes per x-plane than a 16-bit step reaches
+    keeps the 32-bit table: the pack reports it and LbmBatch falls back."""
+    import torch
+    from prtlb import _lib
+    from prtlb.batch import LbmBatch
+    from prtlb.device import ptr, stream_ptr
+    g = np.full((3, 200, 200), prtlb.PORE, np.uint8)     # 40 000 fluid nodes per plane
+    g[:, 0, :] = prtlb.SOLID
+    dev = torch.device("cuda:0")
+    fidx = torch.from_numpy(np.flatnonzero(g.reshape(-1) == prtlb.PORE).astype(np.int32)).to(dev)
+    nf = fidx.numel()
+    lk = _lib.lib().prt_lbm_link_stride(nf)
+    link = torch.empty(18 * lk, dtype=torch.int32, device=dev)
+    work = torch.empty(g.size, dtype=torch.int32, device=dev)
+    s = stream_ptr(dev)
+    _lib.call("prt_lbm_link_table", 19, *g.shape, ptr(fidx), nf, ptr(work), ptr(link), s)
+    l16 = torch.empty(18 * lk, dtype=torch.int16, device=dev)
+    over = torch.zeros(1, dtype=torch.int32, device=dev)
+    _lib.call("prt_lbm_link_pack16", 19, nf, ptr(link), ptr(l16), ptr(over), s)
+    assert int(over.item()) == 1
+    b = LbmBatch(19, storage="fp32")
+    m = b.add(0, g, force=1e-6)
+    assert m.link_bits == 32 and m.link.dtype == torch.int32
+    b2 = LbmBatch(19, storage="fp32")
+    m2 = b2.add(0, prtlb.sphere_pack((24, 20, 16), (2.0, 4.0), 0.5, seed=5), force=1e-6)
+    assert m2.link_bits == 16 and m2.link.dtype == torch.int16
 
 
 @pytest.mark.parametrize("storage", ["fp64", "fp32"])
