@@ -341,6 +341,47 @@ def campaign_study(n_geom=64, edge=(48, 160), seed=0, adr_steps=None, max_steps=
             "adr_steps_mean": float(np.mean([rs[k]["adr_steps"] for k in rs]))}
 
 
+def features_study(edge=128, cpu_edge=48, buf=10):
+    """The per-rock flow features (harmonic pressure and gradient, MIS, UPRM,
+    dw2) on a seeded sphere pack: GPU wall time per map (synchronised, mask
+    resident), and the same maps from the NumPy/SciPy oracle on a smaller
+    pack on one host core."""
+    import numpy as np
+    import torch
+    import prtlb
+    from prtlb import features as Fe
+    out = {"gpu_edge": edge, "cpu_edge": cpu_edge, "buffer": buf}
+
+    def wall(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    g = prtlb.sphere_pack((edge, edge, edge), (4.0, 8.0), 0.35, seed=0) == prtlb.PORE
+    t = torch.from_numpy(g.astype(np.uint8)).cuda()
+    Fe.harmonic_pressure_device(t)                          # warm-up
+    (P, dropped, iters), s = wall(lambda: Fe.harmonic_pressure_device(t))
+    out["harmonic_pressure_s"] = round(s, 4)
+    out["harmonic_cg_iterations"] = iters
+    out["harmonic_pore_voxels"] = int(g.sum()) - dropped
+    out["harmonic_gradient_s"] = round(wall(lambda: Fe.harmonic_gradient_device(t, P=P))[1], 5)
+    out["mis_s"] = round(wall(lambda: Fe.mis_map_device(t, buf=buf))[1], 4)
+    out["uprm_s"] = round(wall(lambda: Fe.uprm_map_device(t))[1], 4)
+    out["dw2_s"] = round(wall(lambda: Fe.dw2_map_device(t))[1], 5)
+    from oracle import features_oracle as O
+    gc = prtlb.sphere_pack((cpu_edge,) * 3, (4.0, 8.0), 0.35, seed=0) == prtlb.PORE
+    for name, fn in (("harmonic_pressure", lambda: O.harmonic_pressure(gc)),
+                     ("mis", lambda: O.mis_map(gc, buf=buf)), ("uprm", lambda: O.uprm_map(gc)),
+                     ("dw2", lambda: O.dw2_map(gc))):
+        t0 = time.perf_counter()
+        fn()
+        out["cpu_%s_s" % name] = round(time.perf_counter() - t0, 3)
+    out["cpu_note"] = "oracle on %d^3, one core (scipy spsolve, binary_dilation, heapq)" % cpu_edge
+    return out
+
+
 def serve_study(n_req=48, rate=4.0, seed=0, budget=200e9, max_steps=100000, transport_cap=None):
     """Serving: Poisson arrivals (rate per s) of solve requests, 30% D2Q9
     (256^2..1024^2 disk packs), 70% D3Q19 (64^3..256^3 sphere packs), Pe and
@@ -386,6 +427,8 @@ def main():
                     help="transport step cap of the campaign and serving studies")
     ap.add_argument("--adr-only", action="store_true",
                     help="only the transport step (128^3 and 256^3)")
+    ap.add_argument("--features", action="store_true",
+                    help="only the flow-feature maps (harmonic pressure, MIS, UPRM, dw2)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -413,6 +456,9 @@ def main():
         return
     if args.precision:
         print(json.dumps({"precision": precision_study()}), flush=True)
+        return
+    if args.features:
+        print(json.dumps({"features": features_study()}), flush=True)
         return
     if args.campaign:
         print(json.dumps({"campaign": campaign_study(args.campaign, adr_steps=args.adr_steps,

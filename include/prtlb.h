@@ -407,6 +407,52 @@ int prt_geodesic(const uint8_t *g, int nx, int ny, int nz, int pore_code,
 int prt_edt(const uint8_t *g, int nx, int ny, int nz, int pore_code,
             int32_t *work_a, int32_t *work_b, float *out, void *stream);
 
+/* --------------------------------------------- harmonic pressure prior */
+/* Laplace's equation on the pore (byte mask, 1 = pore) face-connected to
+ * the inlet: P = 1 on x = 0, 0 on x = nx - 1, no flux through walls; 0 on
+ * solid and on pore the inlet does not reach.  Replaces harmonic_pressure
+ * (3D/tools/harmonic_pressure.py:91-146; its scipy spsolve becomes
+ * Jacobi-preconditioned CG in FP64 to relative residual tol, then float32).
+ * A 2D grid is passed as (nx, 1, ny).  work: prt_harmonic_work_size()
+ * bytes of device scratch.  *dropped_host (if non-NULL): the unreached pore
+ * voxels; *iters_host: CG iterations.  Synchronises the stream. */
+int64_t prt_harmonic_work_size(int nx, int ny, int nz);
+int prt_harmonic_pressure(const uint8_t *pore, int nx, int ny, int nz,
+                          void *work, int64_t work_bytes, double tol,
+                          int max_iter, float *out, int *dropped_host,
+                          int *iters_host, void *stream);
+/* Its gradient (harmonic_gradient, :148-197): out (nd, n) float32, central
+ * differences where both neighbours are pore, one-sided where one is, 0
+ * where none is; with normalise, divided by the largest |component| over
+ * the pore.  scratch: ceil(n / 256) + 1 floats. */
+int prt_harmonic_gradient(const uint8_t *pore, int nx, int ny, int nz, int nd,
+                          const float *P, int normalise, float *out,
+                          float *scratch, void *stream);
+
+/* --------------------------------------------- flow descriptors */
+/* Pore descriptors of 3D/tools/flow_features.py, bit-identical float32
+ * maps; pore is a byte mask (1 = pore), edt its prt_edt distances
+ * (pore_code 1); a 2D grid is (nx, 1, ny).
+ * UPRM (:205-257): the widest path from the inlet; best: n floats;
+ * scratch1: one device int.  Synchronises the stream. */
+int prt_uprm(const uint8_t *pore, int nx, int ny, int nz, const float *edt,
+             float *best, int *scratch1, int max_sweeps, int *sweeps_host,
+             void *stream);
+/* dw2 (:260-280): clip((edt^2 - lo) / span, 0, 1) on the pore, float32. */
+int prt_dw2(const uint8_t *pore, int64_t n, const float *edt, float lo,
+            float span, float *out, void *stream);
+/* Local thickness (:128-150) with the sweep numpy.arange(rmax, 0, -r_step),
+ * rmax the largest edt; rc_work: n doubles. */
+int prt_local_thickness(const uint8_t *pore, int nx, int ny, int nz,
+                        const float *edt, double rmax, double r_step,
+                        double *rc_work, float *out, void *stream);
+/* Painting into the buffers (:153-177) and the MIS finish (:180-201): full
+ * holds the interior slab's local thickness (x in [buf, nx - buf)) and is
+ * completed in place; vmax, vmin: the largest and smallest positive
+ * interior values. */
+int prt_paint_buffers(const uint8_t *pore, int nx, int ny, int nz, int buf,
+                      float vmax, float vmin, float *full, void *stream);
+
 /* ------------------------------------------------ check_physics passes */
 /* The data passes of check_physics (3D/tools/prtlb_3d.py:1632-1843) over
  * device snapshots out (n_t, ns, n_vox) float32 with a pore byte mask

@@ -77,4 +77,10 @@ __device__ inline double block_sum(double v, double *lds) {
   return s;  // valid in thread 0
 }
 
+// The pore voxels (g == pore_code) face-connected to a pore voxel on x = 0,
+// as reach[n] = 1, by repeated face dilation inside the pore (prt_geom.hip).
+// changed: one device int of scratch.  Host code; synchronises the stream.
+int inlet_reach(const uint8_t *g, int X, int Y, int Z, uint8_t pore_code, uint8_t *reach, int *changed,
+                int max_sweeps, hipStream_t st);
+
 }  // namespace prt

@@ -258,6 +258,33 @@ __global__ __launch_bounds__(256) void edt_sqrt_kernel(const int *D, long long N
 
 static unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
 
+int inlet_reach(const uint8_t *g, int X, int Y, int Z, uint8_t pore_code, uint8_t *reach, int *changed,
+                int max_sweeps, hipStream_t st) {
+  const long long N = (long long)X * Y * Z, plane = (long long)Y * Z;
+  reach_init_kernel<<<nblk(N), 256, 0, st>>>(g, pore_code, X, plane, reach);
+  int rc = launch_status("reach init");
+  if (rc) return rc;
+  // sweeps in chunks of 16 between host checks of the change flag; each
+  // sweep only ever sets voxels, so the fixed point is the reachable set
+  for (int done = 0; done < max_sweeps;) {
+    hipError_t e = hipMemsetAsync(changed, 0, sizeof(int), st);
+    if (e != hipSuccess) return hip_fail(e, "inlet reach memset");
+    for (int k = 0; k < 16; ++k) reach_grow_kernel<<<nblk(N), 256, 0, st>>>(g, pore_code, X, Y, Z, reach, changed);
+    if ((rc = launch_status("reach grow"))) return rc;
+    done += 16;
+    int h = 0;
+    e = hipMemcpyAsync(&h, changed, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "inlet reach flag copy");
+    if (!h) break;
+    if (done >= max_sweeps) {
+      set_error("inlet reach: not converged in %d sweeps", max_sweeps);
+      return PRT_E_RANGE;
+    }
+  }
+  return PRT_OK;
+}
+
 static int dims3(int nx, int ny, int nz, int &X, int &Y, int &Z) {
   if (nx < 1 || ny < 1 || nz < 1) {
     set_error("grid dimensions must be positive");
@@ -323,29 +350,9 @@ extern "C" int prt_keep_inlet_connected(uint8_t *g, int nx, int ny, int nz, int 
     return PRT_E_ARG;
   }
   hipStream_t st = (hipStream_t)stream;
-  const long long N = (long long)X * Y * Z, plane = (long long)Y * Z;
-  reach_init_kernel<<<nblk(N), 256, 0, st>>>(g, (uint8_t)pore_code, X, plane, reach);
-  if ((rc = launch_status("reach init"))) return rc;
+  const long long N = (long long)X * Y * Z;
   int *changed = scratch2, *removed = scratch2 + 1;
-  // sweeps in chunks of 16 between host checks of the change flag; each
-  // sweep only ever sets voxels, so the fixed point is the reachable set
-  for (int done = 0; done < max_sweeps;) {
-    hipError_t e = hipMemsetAsync(changed, 0, sizeof(int), st);
-    if (e != hipSuccess) return hip_fail(e, "prt_keep_inlet_connected memset");
-    for (int k = 0; k < 16; ++k)
-      reach_grow_kernel<<<nblk(N), 256, 0, st>>>(g, (uint8_t)pore_code, X, Y, Z, reach, changed);
-    if ((rc = launch_status("reach grow"))) return rc;
-    done += 16;
-    int h = 0;
-    e = hipMemcpyAsync(&h, changed, sizeof(int), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return hip_fail(e, "prt_keep_inlet_connected flag copy");
-    if (!h) break;
-    if (done >= max_sweeps) {
-      set_error("prt_keep_inlet_connected: not converged in %d sweeps", max_sweeps);
-      return PRT_E_RANGE;
-    }
-  }
+  if ((rc = inlet_reach(g, X, Y, Z, (uint8_t)pore_code, reach, changed, max_sweeps, st))) return rc;
   hipError_t e = hipMemsetAsync(removed, 0, sizeof(int), st);
   if (e != hipSuccess) return hip_fail(e, "prt_keep_inlet_connected memset");
   reach_apply_kernel<<<nblk(N), 256, 0, st>>>(g, (uint8_t)pore_code, (uint8_t)solid_code, reach, N, removed);

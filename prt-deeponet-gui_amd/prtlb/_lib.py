@@ -119,6 +119,13 @@ _SIGS = {
                                      _P]),
     "prt_geodesic": (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _I, _P, _P, _P]),
     "prt_edt": (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P]),
+    "prt_harmonic_work_size": (_I64, [_I, _I, _I]),
+    "prt_harmonic_pressure": (_I, [_P, _I, _I, _I, _P, _I64, _D, _I, _P, _P, _P, _P]),
+    "prt_harmonic_gradient": (_I, [_P, _I, _I, _I, _I, _P, _I, _P, _P, _P]),
+    "prt_uprm": (_I, [_P, _I, _I, _I, _P, _P, _P, _I, _P, _P]),
+    "prt_dw2": (_I, [_P, _I64, _P, ctypes.c_float, ctypes.c_float, _P, _P]),
+    "prt_local_thickness": (_I, [_P, _I, _I, _I, _P, _D, _D, _P, _P, _P]),
+    "prt_paint_buffers": (_I, [_P, _I, _I, _I, _I, ctypes.c_float, ctypes.c_float, _P, _P]),
     "prt_check_blocks": (_I, [_I64]),
     "prt_check_nonfinite": (_I, [_P, _I64, _I, _P, _P]),
     "prt_check_pore_range": (_I, [_P, _I, _I, _I64, _P, _I, _P, _P, _P]),

@@ -28,6 +28,7 @@ SOURCES = {
     "prt_geom.hip": ["-ffp-contract=off"],
     "prt_check.hip": ["-ffp-contract=off"],
     "prt_handle.hip": [],
+    "prt_features.hip": ["-ffp-contract=off"],
 }
 
 
