@@ -499,6 +499,11 @@ def main():
             log(st, "256^3 ...")
             extras["d3q19_256_" + st] = time_lbm(g256, st, max(args.steps // 2, 20), args.warmup)
         del g256
+        log("D2Q9 2048^2 ...")
+        g2d = prtlb.disk_pack((2048, 2048), 6.0, 0.6, seed=0, device="cuda")
+        for st in ("fp64", "fp32"):
+            extras["d2q9_2048sq_" + st] = time_lbm(g2d, st, args.steps, args.warmup)
+        del g2d
         log("transport 128^3 ...")
         extras["adr_128_monod_f32"] = time_adr(g128, args.steps, args.warmup)
         log("stream copy ...")
