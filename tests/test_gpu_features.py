@@ -100,3 +100,24 @@ def test_all_features_dict():
     assert set(f) == {"mis", "uprm", "dw2", "dw2_range"}
     assert np.array_equal(f["mis"], F.mis_map(pore, buf=4))
     assert np.array_equal(f["uprm"], F.uprm_map(pore))
+
+
+def test_extruded_2d_volume_reproduces_the_2d_maps_plane_for_plane():
+    """A 3D volume built by repeating a 2D mask along the last axis has, in
+    every plane, the 2D answer (no wall along that axis): the check the
+    reference's features were held to for their 3D generalisation."""
+    rng = np.random.default_rng(11)
+    p2 = rng.random((48, 30)) > 0.35
+    p3 = np.repeat(p2[:, :, None], 5, axis=2)
+    u2, u3 = features.uprm_map(p2), features.uprm_map(p3)
+    m2, m3 = features.mis_map(p2, buf=6), features.mis_map(p3, buf=6)
+    d2, r2 = features.dw2_map(p2)
+    d3, r3 = features.dw2_map(p3)
+    P2, _ = features.harmonic_pressure(p2)
+    P3, _ = features.harmonic_pressure(p3)
+    assert r2 == r3
+    for k in range(5):
+        assert np.array_equal(u3[:, :, k], u2)
+        assert np.array_equal(m3[:, :, k], m2)
+        assert np.array_equal(d3[:, :, k], d2)
+        assert np.abs(P3[:, :, k] - P2).max() < 2e-6
