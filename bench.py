@@ -427,6 +427,9 @@ def main():
                     help="transport step cap of the campaign and serving studies")
     ap.add_argument("--adr-only", action="store_true",
                     help="only the transport step (128^3 and 256^3)")
+    ap.add_argument("--fluid-list", action="store_true",
+                    help="only the fluid-list step on the 128^3 pack (FP32 with 16- and 32-bit "
+                         "links, FP64)")
     ap.add_argument("--features", action="store_true",
                     help="only the flow-feature maps (harmonic pressure, MIS, UPRM, dw2)")
     args = ap.parse_args()
@@ -459,6 +462,14 @@ def main():
         return
     if args.features:
         print(json.dumps({"features": features_study()}), flush=True)
+        return
+    if args.fluid_list:
+        g = prtlb.sphere_pack((128, 128, 128), (4.0, 8.0), 0.35, seed=0)
+        res = {}
+        for name, st, l16 in (("fp32_link16", "fp32", True), ("fp32_link32", "fp32", False),
+                              ("fp64_link16", "fp64", True)):
+            res[name] = time_fluid_list(g, st, args.steps, args.warmup, link16=l16)
+        print(json.dumps({"fluid_list": res}), flush=True)
         return
     if args.campaign:
         print(json.dumps({"campaign": campaign_study(args.campaign, adr_steps=args.adr_steps,
