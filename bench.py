@@ -341,6 +341,57 @@ def campaign_study(n_geom=64, edge=(48, 160), seed=0, adr_steps=None, max_steps=
             "adr_steps_mean": float(np.mean([rs[k]["adr_steps"] for k in rs]))}
 
 
+def geometry_study(edges=(128, 256)):
+    """The geometry and check passes on the GPU against their host forms on
+    the same grids: seeded sphere pack + inlet connectivity, biofilm coating,
+    geodesic distance, distance transform, and check_physics over a
+    synthetic 10-snapshot, 2-chemical result.  Wall seconds, synchronised."""
+    import numpy as np
+    import torch
+    import prtlb
+    from prtlb import geometry as G
+    from scipy import ndimage
+    from oracle import prtlb_oracle as O
+
+    def wall(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    out = {}
+    for E in edges:
+        shape = (E, E, E)
+        r = {}
+        prtlb.sphere_pack((32, 32, 32), (4.0, 8.0), 0.35, seed=1, device="cuda")   # warm-up
+        g, r["pack_gpu_s"] = wall(lambda: prtlb.sphere_pack(shape, (4.0, 8.0), 0.35, seed=0,
+                                                             device="cuda"))
+        _, r["pack_host_s"] = wall(lambda: prtlb.sphere_pack(shape, (4.0, 8.0), 0.35, seed=0))
+        t = torch.from_numpy(g).cuda()
+        _, r["coat_gpu_s"] = wall(lambda: G.coat_with_biofilm_device(t.clone(), 1, 1))
+        _, r["coat_host_s"] = wall(lambda: G.coat_with_biofilm(g, 1, 1))
+        (_, sweeps), r["geodesic_gpu_s"] = wall(lambda: G.geodesic_device(t))
+        r["geodesic_sweeps"] = sweeps
+        _, r["edt_gpu_s"] = wall(lambda: G.edt_device(t))
+        _, r["edt_scipy_s"] = wall(lambda: ndimage.distance_transform_edt(g == prtlb.PORE))
+        rng = np.random.default_rng(0)
+        snaps = torch.from_numpy(rng.random((10, 2) + shape, dtype=np.float32)).cuda()
+        _, r["check_physics_gpu_s"] = wall(lambda: prtlb.check_physics_device(snaps, g))
+        if E <= 128:
+            host = snaps.cpu().numpy()
+            _, r["check_physics_host_s"] = wall(lambda: prtlb.check_physics(host, g))
+        out["edge_%d" % E] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+        del t, snaps
+        torch.cuda.empty_cache()
+    # the oracle's geodesic relaxation is NumPy passes over the grid: 64^3 only
+    g64 = prtlb.sphere_pack((64, 64, 64), (4.0, 8.0), 0.35, seed=0)
+    (_, sw64), gpu64 = wall(lambda: G.geodesic_device(g64))
+    _, host64 = wall(lambda: O.geodesic(g64))
+    out["geodesic_64"] = {"gpu_s": round(gpu64, 4), "oracle_s": round(host64, 3), "sweeps": sw64}
+    return out
+
+
 def features_study(edge=128, cpu_edge=48, buf=10):
     """The per-rock flow features (harmonic pressure and gradient, MIS, UPRM,
     dw2) on a seeded sphere pack: GPU wall time per map (synchronised, mask
@@ -430,6 +481,8 @@ def main():
     ap.add_argument("--fluid-list", action="store_true",
                     help="only the fluid-list step on the 128^3 pack (FP32 with 16- and 32-bit "
                          "links, FP64)")
+    ap.add_argument("--geometry", action="store_true",
+                    help="only the geometry and check passes, GPU against host")
     ap.add_argument("--features", action="store_true",
                     help="only the flow-feature maps (harmonic pressure, MIS, UPRM, dw2)")
     args = ap.parse_args()
@@ -462,6 +515,9 @@ def main():
         return
     if args.features:
         print(json.dumps({"features": features_study()}), flush=True)
+        return
+    if args.geometry:
+        print(json.dumps({"geometry": geometry_study()}), flush=True)
         return
     if args.fluid_list:
         g = prtlb.sphere_pack((128, 128, 128), (4.0, 8.0), 0.35, seed=0)
