@@ -219,6 +219,12 @@ typedef struct prt_adr_params {
   int32_t pinned[8];     /* 1 = mobile with a feed: held at fed[j] on x = 0 */
   double fed[8];
   double dface[8][4];    /* harmonic face diffusivity, [j][2*bf_self + bf_nb] */
+  int32_t immobile_list; /* pore-list launches: the rows of immobile
+                            chemicals hold the pore voxels in list order
+                            (entry k at k) instead of box order; their
+                            loads and stores are then contiguous.  The
+                            dense launches refuse it. */
+  int32_t reserved;
 } prt_adr_params;
 
 /* One explicit step of the flux-form MUSCL-minmod transport plus reactions:

@@ -88,6 +88,10 @@ static int build_args(const prt_adr_params *p, int nx, int ny, int nz, const uin
   }
   const int rc = adr_args_from_params<R>(p, nx, ny, nz, a, grid, block);
   if (rc) return rc;
+  if (p->immobile_list) {
+    set_error("prt_adr_step: immobile_list needs the pore-list launch");
+    return PRT_E_ARG;
+  }
   a.flags = flags; a.u = u; a.cin = cin; a.cout = cout;
   return PRT_OK;
 }

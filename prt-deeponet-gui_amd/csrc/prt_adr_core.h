@@ -22,6 +22,7 @@ struct AdrArgs {
   long long N;
   int ns, biotic_on, coupled, abio_index, abio_target, abio_product, surface_only;
   int sorption, sorb_c, sorb_s, write_solid;
+  int imm_list;  // immobile chemicals stored in pore-list order (list launches)
   R dt, k, ka, kd, ks_d, ks_a, alpha_dt, neg_alpha, c2, yield_, c3_init, k_sorb, k_desorb;
   int alpha_pos;
   FastDiv div_yz, div_z;                 // flat index -> (x, y, z)
@@ -193,10 +194,14 @@ __device__ inline void geom_table(const AdrArgs<R> &a, uint32_t n, uint32_t w, c
   }
 }
 
-// The step at one pore voxel n with geometry g: writes the new values to
-// cout and returns them in cnew.
+// The step at one pore voxel n (list entry k) with geometry g: writes the
+// new values to cout and returns them in cnew.  An immobile chemical is
+// only ever read and written at the voxel itself, so with a.imm_list its
+// row holds the pore voxels in list order (entry k at k, rows of mobile
+// chemicals stay box arrays for the stencil): its load and store are then
+// contiguous across a wave instead of scattered over the box.
 template <typename R, int ND, int NSM>
-__device__ inline void adr_update(const AdrArgs<R> &a, uint32_t n, const VoxGeom<R, ND> &g, R *cnew) {
+__device__ inline void adr_update(const AdrArgs<R> &a, uint32_t n, uint32_t k, const VoxGeom<R, ND> &g, R *cnew) {
 #pragma clang fp contract(off)
   const int ns = a.ns;
   const uint32_t w = g.w;
@@ -210,8 +215,11 @@ __device__ inline void adr_update(const AdrArgs<R> &a, uint32_t n, const VoxGeom
     c[j] = R(0);
     if (j >= ns) continue;
     const R *cj = a.cin + (size_t)j * a.N;
+    if (!a.mobile[j]) {
+      c[j] = ld32(cj, a.imm_list ? k : n);
+      continue;
+    }
     c[j] = ld32(cj, n);
-    if (!a.mobile[j]) continue;
 #pragma unroll
     for (int ax = 0; ax < ND; ++ax)
 #pragma unroll
@@ -362,7 +370,7 @@ __device__ inline void adr_update(const AdrArgs<R> &a, uint32_t n, const VoxGeom
     if (j >= ns) continue;
     const R t = a.mobile[j] ? tr[j] : R(0);
     const R v = npmax(c[j] + a.dt * (t + r[j]), R(0));
-    st32(a.cout + (size_t)j * a.N, n, v);
+    st32(a.cout + (size_t)j * a.N, (!a.mobile[j] && a.imm_list) ? k : n, v);
     cnew[j] = v;
   }
 }
@@ -381,7 +389,7 @@ __device__ inline bool adr_voxel(const AdrArgs<R> &a, uint32_t n, int x, int y, 
   }
   VoxGeom<R, ND> g;
   geom_dense<R, ND>(a, n, fl, x, y, z, g);
-  adr_update<R, ND, NSM>(a, n, g, cnew);
+  adr_update<R, ND, NSM>(a, n, n, g, cnew);
   return true;
 }
 
@@ -395,7 +403,7 @@ __device__ inline bool adr_entry(const AdrArgs<R> &a, const int32_t *pidx, const
   const uint32_t n = (uint32_t)ldtab<NTT>(pidx, k);
   VoxGeom<R, ND> g;
   geom_table<R, ND, NTT>(a, n, w, face, k, n_pore, g);
-  adr_update<R, ND, NSM>(a, n, g, cnew);
+  adr_update<R, ND, NSM>(a, n, k, g, cnew);
   return true;
 }
 
@@ -472,6 +480,7 @@ inline int adr_args_from_params(const prt_adr_params *p, int nx, int ny, int nz,
   a.abio_index = p->abio_index; a.abio_target = p->abio_target; a.abio_product = p->abio_product;
   a.surface_only = p->surface_only; a.sorption = p->sorption; a.sorb_c = p->sorb_c; a.sorb_s = p->sorb_s;
   a.write_solid = p->write_solid;
+  a.imm_list = p->immobile_list;
   a.dt = (R)p->dt; a.k = (R)p->k; a.ka = (R)p->ka; a.kd = (R)p->kd;
   a.ks_d = (R)p->ks_donor; a.ks_a = (R)p->ks_acceptor; a.alpha_dt = (R)p->alpha_dt;
   a.neg_alpha = (R)p->neg_alpha; a.c2 = (R)p->c2; a.yield_ = (R)p->yield_; a.c3_init = (R)p->c3_init;

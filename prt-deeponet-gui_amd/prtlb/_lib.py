@@ -51,6 +51,7 @@ class AdrParams(ctypes.Structure):
         ("k_desorb", ctypes.c_double),
         ("mobile", ctypes.c_int32 * 8), ("pinned", ctypes.c_int32 * 8),
         ("fed", ctypes.c_double * 8), ("dface", (ctypes.c_double * 4) * 8),
+        ("immobile_list", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 

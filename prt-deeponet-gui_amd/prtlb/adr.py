@@ -266,7 +266,57 @@ class TransportSolver:
         self._plan = None
         self.means_scratch = None
         self.use_graph = self.n < (1 << 20)
+        # immobile chemicals are read and written at their own voxel only:
+        # the pore-list launch keeps their rows in list order (contiguous)
+        self._immobile = [j for j in range(self.ns) if not self.params.mobile[j]]
+        self._pore_list = False
         self.pore_list = True
+
+    @property
+    def pore_list(self):
+        """True: one thread per pore voxel (the default); False: one thread per
+        box voxel.  Switching converts the immobile rows' layout."""
+        return self._pore_list
+
+    @pore_list.setter
+    def pore_list(self, on):
+        on = bool(on)
+        if on == self._pore_list:
+            return
+        box = [self._to_box(c) for c in self.c] if self._pore_list else list(self.c)
+        self._pore_list = on
+        self.params.immobile_list = int(on and bool(self._immobile))
+        self.c = [self._from_box(c) for c in box] if on else box
+        self._graph_key = None
+
+    def _from_box(self, c):
+        """(ns, n) box field -> the layout the current launch reads."""
+        if not (self._pore_list and self._immobile):
+            return c
+        out = c.clone()
+        for j in self._immobile:
+            row = c[j][self.pore_idx.long()]
+            out[j].zero_()
+            out[j][:self.n_pore] = row
+        return out
+
+    def _to_box(self, c):
+        """The launch's layout -> an (ns, n) box field."""
+        if not (self._pore_list and self._immobile):
+            return c
+        out = c.clone()
+        for j in self._immobile:
+            row = c[j][:self.n_pore].clone()
+            out[j].zero_()
+            out[j][self.pore_idx.long()] = row
+        return out
+
+    def load_state(self, c):
+        """Set both buffers to the (ns, n) box field c."""
+        c = torch.as_tensor(c, dtype=self.c[0].dtype).to(self.dev).reshape(self.ns, self.n)
+        v = self._from_box(c)
+        self.c[0].copy_(v)
+        self.c[1].copy_(v)
 
     @staticmethod
     def _surface_sum(m, R):
@@ -286,8 +336,9 @@ class TransportSolver:
         return stream_ptr(self.dev)
 
     def current(self):
-        """The newest concentration field, (ns, n) on the device."""
-        return self.c[self.latest]
+        """The newest concentration field, (ns, n) on the device (a box
+        field; a copy when immobile rows are held in list order)."""
+        return self._to_box(self.c[self.latest])
 
     GRAPH_STEPS = 32
 

@@ -43,8 +43,7 @@ def _evolve(backend, g, v, pe, da, ns, chem, steps, c0=None, da_abio=0.0):
     import prtlb
     T = prtlb.TransportSolver(g, v, pe, da, ns, chem, da_abio)
     if c0 is not None:
-        T.current().copy_(torch.from_numpy(np.ascontiguousarray(c0, np.float32)).to(T.dev))
-        T.c[1 - T.latest].copy_(T.current())
+        T.load_state(torch.from_numpy(np.ascontiguousarray(c0, np.float32)))
     T.step_n(steps)
     return T.current().cpu().numpy(), T.dt, T.m
 
