@@ -65,6 +65,24 @@ def test_vs_oracle_3d(ns, da, da_abio, ks):
     assert np.array_equal(out, outo) and np.array_equal(tn, tno)
 
 
+@pytest.mark.parametrize("shape", [(1, 7, 9), (6, 5, 1), (2, 2, 70), (3, 40), (40, 1)])
+def test_thin_grids_vs_oracle(shape):
+    """Degenerate extents -- a one-plane grid (the inlet pin then reaches
+    every voxel), one-voxel y or z, short rows -- bit for bit against the
+    oracle with every chemistry kind (4 species: mobile and immobile rows)."""
+    rng = np.random.default_rng(sum(shape) + 1)
+    g = np.where(rng.random(shape) < 0.2, prtlb.SOLID, prtlb.PORE).astype(np.uint8)
+    g[0] = prtlb.PORE
+    rngv = np.random.default_rng(7)
+    v = np.zeros((3,) + shape, np.float32)
+    v[0] = np.where(g == prtlb.PORE, 0.5 + rngv.random(shape), 0.0)
+    print("case thin", shape, flush=True)
+    kw = dict(pe=10.0, da=1.0, n_t=3, n_species=4, steps=150, da_abio=0.5)
+    out, tn = prtlb.solve_adr(g, v, **kw)
+    outo, tno = O.solve_adr(g, v, **kw)
+    assert np.array_equal(out, outo) and np.array_equal(tn, tno)
+
+
 def test_auto_stop_matches_oracle():
     # the stop step rests on float32 pore means in NumPy's summation order
     g = prtlb.disk_pack((48, 24), 3.0, 0.7, seed=3)

@@ -57,6 +57,22 @@ def test_vs_oracle_fixed_steps(shape, seed):
     assert np.array_equal(v64s, v64)
 
 
+@pytest.mark.parametrize("shape", [(6, 5, 1), (1, 7, 9), (9, 1, 8), (3, 40), (2, 2, 70)])
+@pytest.mark.parametrize("storage", ["fp64", "fp32"])
+def test_thin_grids_vs_oracle(shape, storage):
+    """Degenerate extents: one-node axes (each node is its own periodic
+    neighbour) and rows shorter or longer than a cache line of populations,
+    which exercise the row padding of the population layout."""
+    rng = np.random.default_rng(sum(shape))
+    g = np.where(rng.random(shape) < 0.25, prtlb.SOLID, prtlb.PORE).astype(np.uint8)
+    g[0] = prtlb.PORE
+    print("case", shape, storage, flush=True)
+    _, (v64o, _) = O.stokes(g, nit=60, force=1e-5, tol=0.0, return_f64=True)
+    _, v64 = solver(g)(g, nit=60, force=1e-5, tol=0.0, return_f64=True, storage=storage)
+    tol = 1e-12 if storage == "fp64" else 1e-4
+    assert rel(v64, v64o) < tol
+
+
 def test_reduced_precision_errors():
     g = prtlb.disk_pack((64, 48), 4.0, 0.6, seed=5)
     ref64 = prtlb.stokes_d2q9(g, nit=20000, force=2e-6, tol=1e-10, return_f64=True)[1]
