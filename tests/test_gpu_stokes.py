@@ -109,6 +109,16 @@ def test_poiseuille_2d_3d():
     ux, ex = v[0][6, 1:-1], parabola(20, 2e-6, 0.8)
     assert abs(ux.max() / ex.max() - 1) < 0.01
     assert np.abs(ux - ex).max() / ex.max() < 0.015
+    # a least-squares parabola through the profile
+    # (3D/tools/test_flow_solvers.py:245-262): its curvature -g/(2 nu) gives
+    # the viscosity (tau - 1/2)/3, its roots the no-slip walls half a voxel
+    # beyond the last fluid nodes, and nothing is left over after the fit
+    y = np.arange(1, 21) - 0.5
+    c = np.polyfit(y, ux, 2)
+    assert abs(-2e-6 / (2 * c[0]) / 0.1 - 1) < 0.01, c
+    lo, hi = np.sort(np.roots(c))
+    assert abs(((0.5 - lo) + (hi - 19.5)) / 2 - 0.5) < 0.1, (lo, hi)
+    assert np.abs(ux - np.polyval(c, y)).max() / ux.max() < 0.01
     v3 = prtlb.stokes_d3q19(_slit3d(16), nit=40000, force=2e-6, tau_lb=0.8)
     ux3, ex3 = v3[0][4, 1:-1, 4], parabola(16, 2e-6, 0.8)
     assert abs(ux3.max() / ex3.max() - 1) < 0.015
