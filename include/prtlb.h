@@ -245,7 +245,7 @@ int prt_adr_step(const prt_adr_params *p, int nx, int ny, int nz,
 /* Pore table of a pore list, built once per transport solve (it depends on
  * the flags and the velocity only).  pore_idx holds n_pore flat voxel indices
  * (C order keeps every later load coalesced); for entry k the call writes
- * word[k] (the open/far/biofilm face bits and edge codes of that voxel) and
+ * word[k] (the reach codes and face bits of that voxel) and
  * its face velocities: face[4k .. 4k+3] = (u_lo, u_hi) of axes 0 and 1, and
  * in 3D face[4 n_pore + 2k .. +1] = (u_lo, u_hi) of axis 2 (face holds
  * 2 nd n_pore values).  An entry that is not a pore voxel, or not below

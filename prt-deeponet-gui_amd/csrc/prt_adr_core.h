@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 
 #include "prt_common.h"
@@ -23,6 +24,7 @@ struct AdrArgs {
   int ns, biotic_on, coupled, abio_index, abio_target, abio_product, surface_only;
   int sorption, sorb_c, sorb_s, write_solid;
   int imm_list;  // immobile chemicals stored in pore-list order (list launches)
+  int mad24;     // every axis stride in bytes fits a signed 24-bit operand
   R dt, k, ka, kd, ks_d, ks_a, alpha_dt, neg_alpha, c2, yield_, c3_init, k_sorb, k_desorb;
   int alpha_pos;
   FastDiv div_yz, div_z;                 // flat index -> (x, y, z)
@@ -48,13 +50,15 @@ __device__ inline R npsign(R d) {
   return keep ? d : __builtin_copysign(R(1), d);
 }
 
-// C_face = up + 1/2 minmod(down - up, up - upup), 3D/tools/prtlb_3d.py:700-705
+// C_face = up + 1/2 minmod(down - up, up - upup), 3D/tools/prtlb_3d.py:700-705.
+// The magnitude is the hardware minimum: it differs from np.minimum only
+// when d1 or d2 is NaN, and then the sign factor is NaN and so is the slope.
 template <typename R>
 __device__ inline R muscl(R up, R upup, R down) {
 #pragma clang fp contract(off)
   const R d1 = down - up;
   const R d2 = up - upup;
-  const R slope = (R(0.5) * (npsign(d1) + npsign(d2))) * npmin(fabs(d1), fabs(d2));
+  const R slope = (R(0.5) * (npsign(d1) + npsign(d2))) * fmin(fabs(d1), fabs(d2));
   return up + R(0.5) * slope;
 }
 
@@ -79,24 +83,35 @@ __device__ inline void st32(T *base, uint32_t i, T v) {
 }
 
 // Pore-table word of one pore voxel (one uint32):
-//   bits 6a .. 6a+5, axis a = 0 .. ND-1: low face open, high face open, far
-//     low face open, far high face open, low neighbour biofilm, high
-//     neighbour biofilm (a face is open when the neighbour across it is pore);
-//   bit 18: the voxel itself is biofilm;
-//   bits 19+4a, 21+4a: edge codes of axis a, low and high side (0 interior,
-//     1 one voxel from the edge, 2 on the edge), which fix the clamped
-//     neighbour offsets;
-//   bit 31: x == 2 (with the x low code, where the inlet pin reaches).
-// 0xFFFFFFFF (edge code 3) marks a list entry that is not a pore voxel.
+//   bits 4a .. 4a+3, axis a = 0 .. ND-1: reach on the low and the high side
+//     (2 bits each), which places the neighbour values the scheme reads:
+//     the second one on a side r strides away and the first min(r, 1);
+//   bits 12+4a .. 12+4a+3: low face open, high face open, low neighbour
+//     biofilm, high neighbour biofilm (a face is open when the neighbour
+//     across it is pore; a face on the box edge is open, towards the voxel
+//     itself, as the edge padding makes it);
+//   bit 24: the voxel itself is biofilm;
+//   bits 25-26: min(x, 3), for the inlet pin;
+//   bits 27+2a, 28+2a, axes a < ND-1: far low face open, far high face open.
+// Along the axes a < ND-1 the reach is the distance to the box edge, capped
+// at 2 (the edge clamp), and the face bits pick among the values read.
+// Along the last axis, whose neighbours share the voxel's cache lines, the
+// reach is cut back to the values the scheme uses, so no select is needed:
+// the first value is the neighbour across the face when that face is open
+// and not on the box edge, else the voxel itself; the second is the next
+// one out when that is pore and inside the box, else the first.
+// 0xFFFFFFFF (reach 3) marks a list entry that is not a pore voxel.
 constexpr uint32_t kPoreWordNone = 0xFFFFFFFFu;
+constexpr int kWordFace = 12, kWordBiofilm = 24, kWordX = 25, kWordFar = 27;
 
-// Geometry of one pore voxel for one step: the clamped neighbour indices,
-// the word above and the face velocities.  Built from the flag and velocity
-// grids (dense launch, table build) or read from the pore table (list
-// launch); both give the same values, so the update is bit-identical.
+// Geometry of one pore voxel for one step: the byte offsets of the
+// neighbour values in an array of R, the word above and the face
+// velocities.  Built from the flag and velocity grids (dense launch, table
+// build) or read from the pore table (list launch); both give the same
+// values, so the update is bit-identical.
 template <typename R, int ND>
 struct VoxGeom {
-  uint32_t nidx[ND][4];  // [axis][m2, m1, p1, p2]
+  uint32_t noff[ND][4];  // [axis][m2, m1, p1, p2], bytes
   uint32_t w;
   R u_lo[ND], u_hi[ND];
 };
@@ -106,59 +121,97 @@ __device__ inline int axis_stride(const AdrArgs<R> &a, int ax3) {
   return ax3 == 0 ? a.Y * a.Z : (ax3 == 1 ? a.Z : 1);
 }
 
-// Clamped neighbour indices from the edge codes of word w.
+// base at a byte offset (a wave-uniform base plus one vector offset).
+template <typename T>
+__device__ inline T ldb(const T *base, uint32_t off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+
+// Neighbour offsets of voxel n from reach codes rc (bits 4a .. 4a+3): the
+// second value on a side sits r strides away and the first min(r, 1).
+// With a.mad24 each offset is one 24-bit multiply-add of the reach and the
+// stride in bytes; otherwise the strides are picked with selects.
 template <typename R, int ND>
-__device__ inline void geom_index(const AdrArgs<R> &a, uint32_t n, VoxGeom<R, ND> &g) {
+__device__ inline void geom_index(const AdrArgs<R> &a, uint32_t n, uint32_t rc, VoxGeom<R, ND> &g) {
+  const uint32_t nb = n * (uint32_t)sizeof(R);
+  // 3D axes x, y, z; a 2D grid is (X, 1, Z) with axes x, z
+  auto sbytes = [&](int ax) { return axis_stride(a, (ND == 2 && ax == 1) ? 2 : ax) * (int)sizeof(R); };
+  if (a.mad24) {
 #pragma unroll
-  for (int ax = 0; ax < ND; ++ax) {
-    // 3D axes x, y, z; a 2D grid is (X, 1, Z) with axes x, z
-    const int stride = axis_stride(a, (ND == 2 && ax == 1) ? 2 : ax);
-    const uint32_t lo = (g.w >> (19 + 4 * ax)) & 3u, hi = (g.w >> (21 + 4 * ax)) & 3u;
-    const int om1 = lo == 2 ? 0 : stride;
-    const int om2 = lo == 0 ? 2 * stride : om1;
-    const int op1 = hi == 2 ? 0 : stride;
-    const int op2 = hi == 0 ? 2 * stride : op1;
-    g.nidx[ax][0] = n - (uint32_t)om2;
-    g.nidx[ax][1] = n - (uint32_t)om1;
-    g.nidx[ax][2] = n + (uint32_t)op1;
-    g.nidx[ax][3] = n + (uint32_t)op2;
+    for (int ax = 0; ax < ND; ++ax) {
+      const int sb = sbytes(ax);
+      const int rl = (rc >> (4 * ax)) & 3u, rh = (rc >> (4 * ax + 2)) & 3u;
+      g.noff[ax][0] = nb + (uint32_t)__mul24(rl, -sb);
+      g.noff[ax][1] = nb + (uint32_t)__mul24(min(rl, 1), -sb);
+      g.noff[ax][2] = nb + (uint32_t)__mul24(min(rh, 1), sb);
+      g.noff[ax][3] = nb + (uint32_t)__mul24(rh, sb);
+    }
+  } else {
+#pragma unroll
+    for (int ax = 0; ax < ND; ++ax) {
+      const int sb = sbytes(ax);
+      const int rl = (rc >> (4 * ax)) & 3u, rh = (rc >> (4 * ax + 2)) & 3u;
+      const int om1 = rl ? sb : 0, op1 = rh ? sb : 0;
+      g.noff[ax][0] = nb - (uint32_t)(rl == 2 ? 2 * sb : om1);
+      g.noff[ax][1] = nb - (uint32_t)om1;
+      g.noff[ax][2] = nb + (uint32_t)op1;
+      g.noff[ax][3] = nb + (uint32_t)(rh == 2 ? 2 * sb : op1);
+    }
   }
 }
 
 // Geometry from the grids: pore voxel n = (x*Y + y)*Z + z with flag byte fl.
+// The neighbours are first taken with the edge clamp alone (reach = the
+// distance to the box edge, capped at 2), which fixes the face bits and
+// velocities; along the last axis the reach is then cut back where a face or
+// the face beyond it is closed.
 template <typename R, int ND>
 __device__ inline void geom_dense(const AdrArgs<R> &a, uint32_t n, uint8_t fl, int x, int y, int z,
                                   VoxGeom<R, ND> &g) {
 #pragma clang fp contract(off)
-  uint32_t w = ((uint32_t)(fl >> 1) & 1u) << 18;
-  if (x == 2) w |= 1u << 31;
+  uint32_t edge = 0;
 #pragma unroll
   for (int ax = 0; ax < ND; ++ax) {
     const int ax3 = (ND == 2 && ax == 1) ? 2 : ax;
     const int ext = ax3 == 0 ? a.X : (ax3 == 1 ? a.Y : a.Z);
     const int p = ax3 == 0 ? x : (ax3 == 1 ? y : z);
-    const uint32_t lo = p == 0 ? 2u : (p == 1 ? 1u : 0u);
-    const uint32_t hi = p == ext - 1 ? 2u : (p == ext - 2 ? 1u : 0u);
-    w |= (lo << (19 + 4 * ax)) | (hi << (21 + 4 * ax));
+    edge |= ((uint32_t)min(p, 2) | (uint32_t)min(ext - 1 - p, 2) << 2) << (4 * ax);
   }
-  g.w = w;
-  geom_index<R, ND>(a, n, g);
+  geom_index<R, ND>(a, n, edge, g);
+  uint32_t w = ((uint32_t)(fl >> 1) & 1u) << kWordBiofilm | (uint32_t)min(x, 3) << kWordX;
 #pragma unroll
   for (int ax = 0; ax < ND; ++ax) {
     uint8_t nf[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) nf[k] = ld32(a.flags, g.nidx[ax][k]);
+    for (int k = 0; k < 4; ++k) nf[k] = ld32(a.flags, g.noff[ax][k] / (uint32_t)sizeof(R));
     const R *ua = a.u + (size_t)ax * a.N;
-    const R um = ld32(ua, g.nidx[ax][1]), uc = ld32(ua, n), up = ld32(ua, g.nidx[ax][2]);
+    const R um = ldb(ua, g.noff[ax][1]), uc = ld32(ua, n), up = ldb(ua, g.noff[ax][2]);
     const bool o_lo = nf[1] & 1, o_hi = nf[2] & 1;
     const bool f_lo = o_lo && (nf[0] & 1);
     const bool f_hi = o_hi && (nf[3] & 1);
     g.u_lo[ax] = o_lo ? R(0.5) * (uc + um) : R(0);
     g.u_hi[ax] = o_hi ? R(0.5) * (uc + up) : R(0);
-    g.w |= ((uint32_t)o_lo | (uint32_t)o_hi << 1 | (uint32_t)f_lo << 2 | (uint32_t)f_hi << 3 |
-            (uint32_t)((nf[1] >> 1) & 1) << 4 | (uint32_t)((nf[2] >> 1) & 1) << 5)
-           << (6 * ax);
+    const uint32_t el = (edge >> (4 * ax)) & 3u, eh = (edge >> (4 * ax + 2)) & 3u;
+    if (ax == ND - 1) {
+      const uint32_t rl = o_lo ? (f_lo ? el : min(el, 1u)) : 0u;
+      const uint32_t rh = o_hi ? (f_hi ? eh : min(eh, 1u)) : 0u;
+      w |= (rl | rh << 2) << (4 * ax);
+    } else {
+      w |= (el | eh << 2) << (4 * ax);
+      w |= ((uint32_t)f_lo | (uint32_t)f_hi << 1) << (kWordFar + 2 * ax);
+    }
+    w |= ((uint32_t)o_lo | (uint32_t)o_hi << 1 | (uint32_t)((nf[1] >> 1) & 1) << 2 |
+          (uint32_t)((nf[2] >> 1) & 1) << 3)
+         << (kWordFace + 4 * ax);
   }
+  g.w = w;
+  // only the last axis changes; its stride is one element
+  const uint32_t nb = n * (uint32_t)sizeof(R), sb = (uint32_t)sizeof(R);
+  const uint32_t rl = (w >> (4 * (ND - 1))) & 3u, rh = (w >> (4 * (ND - 1) + 2)) & 3u;
+  g.noff[ND - 1][0] = nb - rl * sb;
+  g.noff[ND - 1][1] = nb - min(rl, 1u) * sb;
+  g.noff[ND - 1][2] = nb + min(rh, 1u) * sb;
+  g.noff[ND - 1][3] = nb + rh * sb;
 }
 
 // Face velocities of list entry k: axes 0 and 1 as one 4-vector
@@ -177,11 +230,10 @@ __device__ inline T ldtab(const T *base, uint32_t i) {
   else return ld32(base, i);
 }
 
+// Face velocities of list entry k into g (the loads are issued here, their
+// values used later).
 template <typename R, int ND, bool NTT>
-__device__ inline void geom_table(const AdrArgs<R> &a, uint32_t n, uint32_t w, const R *face, uint32_t k,
-                                  uint32_t n_pore, VoxGeom<R, ND> &g) {
-  g.w = w;
-  geom_index<R, ND>(a, n, g);
+__device__ inline void geom_faces(const R *face, uint32_t k, uint32_t n_pore, VoxGeom<R, ND> &g) {
   const vec4<R> f01 = ldtab<NTT>(reinterpret_cast<const vec4<R> *>(face), k);
   g.u_lo[0] = f01.x;
   g.u_hi[0] = f01.y;
@@ -205,9 +257,9 @@ __device__ inline void adr_update(const AdrArgs<R> &a, uint32_t n, uint32_t k, c
 #pragma clang fp contract(off)
   const int ns = a.ns;
   const uint32_t w = g.w;
-  const int bf_self = (w >> 18) & 1;
-  // every neighbour value is loaded up front and unconditionally (indices
-  // are clamped, so always in bounds); the face rules pick among them
+  const int bf_self = (w >> kWordBiofilm) & 1;
+  // every value the stencil may need is loaded up front and unconditionally
+  // (the reach codes keep every offset inside the box)
   R c[NSM], tr[NSM], cn[NSM][ND][4];
 #pragma unroll
   for (int j = 0; j < NSM; ++j) {
@@ -223,23 +275,25 @@ __device__ inline void adr_update(const AdrArgs<R> &a, uint32_t n, uint32_t k, c
 #pragma unroll
     for (int ax = 0; ax < ND; ++ax)
 #pragma unroll
-      for (int k = 0; k < 4; ++k) cn[j][ax][k] = ld32(cj, g.nidx[ax][k]);
+      for (int k = 0; k < 4; ++k) cn[j][ax][k] = ldb(cj, g.noff[ax][k]);
   }
   // the inlet pin (3D/tools/prtlb_3d.py:1398-1400) applied as the values are
   // read: a voxel on x = 0 that is read at all is a pore voxel.  Only voxels
-  // with x <= 2 (or a one-plane grid) can see x = 0.
-  const uint32_t xlo = (w >> 19) & 3u;
-  const bool x_is0 = xlo == 2, x_le1 = xlo != 0, x_le2 = x_le1 || (w >> 31);
-  if (x_le2 || a.X == 1) {
+  // with x <= 2 can see x = 0; along x the values read sit at x - reach,
+  // x - min(reach, 1), x + min(reach, 1) and x + reach.
+  const uint32_t xpos = (w >> kWordX) & 3u;
+  const bool x_is0 = xpos == 0;
+  if (xpos <= 2) {
+    const uint32_t el = w & 3u, eh = (w >> 2) & 3u;
     bool onx0[ND][4];
 #pragma unroll
     for (int ax = 0; ax < ND; ++ax)
 #pragma unroll
       for (int k = 0; k < 4; ++k) onx0[ax][k] = x_is0;
-    onx0[0][0] = x_le2;
-    onx0[0][1] = x_le1;
-    onx0[0][2] = a.X == 1;
-    onx0[0][3] = a.X == 1;
+    onx0[0][0] = xpos == el;
+    onx0[0][1] = xpos == min(el, 1u);
+    onx0[0][2] = x_is0 && eh == 0;
+    onx0[0][3] = x_is0 && eh == 0;
 #pragma unroll
     for (int j = 0; j < NSM; ++j) {
       if (j >= ns || !a.pinned[j]) continue;
@@ -253,28 +307,40 @@ __device__ inline void adr_update(const AdrArgs<R> &a, uint32_t n, uint32_t k, c
     }
   }
 
+  // diffusion coefficients of a face, from the 2x2 table (own and
+  // neighbour biofilm), picked with selects: a runtime index into the
+  // kernel argument block would copy the whole block to scratch memory
+  R t0[NSM], t1[NSM];
+#pragma unroll
+  for (int j = 0; j < NSM; ++j) {
+    t0[j] = bf_self ? a.dface[j][2] : a.dface[j][0];
+    t1[j] = bf_self ? a.dface[j][3] : a.dface[j][1];
+  }
   R divu = R(0);
 #pragma unroll
   for (int ax = 0; ax < ND; ++ax) {
-    const uint32_t fb = w >> (6 * ax);
-    const bool o_lo = fb & 1, o_hi = (fb >> 1) & 1, f_lo = (fb >> 2) & 1, f_hi = (fb >> 3) & 1;
-    const bool bm1 = (fb >> 4) & 1, bp1 = (fb >> 5) & 1;
+    const uint32_t fb = w >> (kWordFace + 4 * ax);
+    const bool o_lo = fb & 1, o_hi = (fb >> 1) & 1, bm1 = (fb >> 2) & 1, bp1 = (fb >> 3) & 1;
+    const uint32_t ff = w >> (kWordFar + 2 * ax);
+    const bool f_lo = ff & 1, f_hi = (ff >> 1) & 1;
+    const bool direct = ax == ND - 1;
     const R u_lo = g.u_lo[ax], u_hi = g.u_hi[ax];
     divu = (ax == 0) ? R(0) + (u_hi - u_lo) : divu + (u_hi - u_lo);
 #pragma unroll
     for (int j = 0; j < NSM; ++j) {
       if (j >= ns || !a.mobile[j]) continue;
+      // along the last axis the reach codes already point a closed face's
+      // values at the voxel (Cm1 = C) and a closed far face's at the first
+      // neighbour (Cm2 = Cm1); along the others the face bits pick
       const R C = c[j];
-      const R Cm1 = o_lo ? cn[j][ax][1] : C;
-      const R Cp1 = o_hi ? cn[j][ax][2] : C;
-      const R Cm2 = f_lo ? cn[j][ax][0] : Cm1;
-      const R Cp2 = f_hi ? cn[j][ax][3] : Cp1;
-      // 2x2 table picked with selects: a runtime index into the kernel
-      // argument block would copy the whole block to scratch memory
-      const R t0 = bf_self ? a.dface[j][2] : a.dface[j][0];
-      const R t1 = bf_self ? a.dface[j][3] : a.dface[j][1];
-      const R d_lo = o_lo ? (bm1 ? t1 : t0) : R(0);
-      const R d_hi = o_hi ? (bp1 ? t1 : t0) : R(0);
+      const R Cm1 = (direct || o_lo) ? cn[j][ax][1] : C;
+      const R Cp1 = (direct || o_hi) ? cn[j][ax][2] : C;
+      const R Cm2 = (direct || f_lo) ? cn[j][ax][0] : Cm1;
+      const R Cp2 = (direct || f_hi) ? cn[j][ax][3] : Cp1;
+      // a closed face's difference is +0 and its coefficient is not zeroed:
+      // t * +0 = 0 * +0 for a finite t >= 0
+      const R d_lo = bm1 ? t1[j] : t0[j];
+      const R d_hi = bp1 ? t1[j] : t0[j];
       const R dif = d_hi * (Cp1 - C) + d_lo * (Cm1 - C);
       tr[j] = (ax == 0) ? R(0) + dif : tr[j] + dif;
       // upwind face values: the upwind side picks the three muscl operands,
@@ -292,7 +358,8 @@ __device__ inline void adr_update(const AdrArgs<R> &a, uint32_t n, uint32_t k, c
   // closed faces over 2*nd, edge-replicated neighbours (:1000-1018)
   int closed = 0;
 #pragma unroll
-  for (int ax = 0; ax < ND; ++ax) closed += !((w >> (6 * ax)) & 1) + !((w >> (6 * ax + 1)) & 1);
+  for (int ax = 0; ax < ND; ++ax)
+    closed += !((w >> (kWordFace + 4 * ax)) & 1) + !((w >> (kWordFace + 4 * ax + 1)) & 1);
 
   // ---- reactions, 3D/tools/prtlb_3d.py:1224-1367
   R r[NSM];
@@ -398,11 +465,16 @@ __device__ inline bool adr_voxel(const AdrArgs<R> &a, uint32_t n, int x, int y, 
 template <typename R, int ND, int NSM, bool NTT = false>
 __device__ inline bool adr_entry(const AdrArgs<R> &a, const int32_t *pidx, const uint32_t *word, const R *face,
                                  uint32_t k, uint32_t n_pore, R *cnew) {
-  const uint32_t w = ldtab<NTT>(word, k);
-  if (w == kPoreWordNone) return false;
-  const uint32_t n = (uint32_t)ldtab<NTT>(pidx, k);
+  // the word, the voxel index and the face velocities are all at entry k
+  // (k < n_pore, in range whatever the word says), so the three loads go out
+  // together instead of one after the other
   VoxGeom<R, ND> g;
-  geom_table<R, ND, NTT>(a, n, w, face, k, n_pore, g);
+  const uint32_t w = ldtab<NTT>(word, k);
+  const uint32_t n = (uint32_t)ldtab<NTT>(pidx, k);
+  geom_faces<R, ND, NTT>(face, k, n_pore, g);
+  if (w == kPoreWordNone || n >= (uint32_t)a.N) return false;
+  g.w = w;
+  geom_index<R, ND>(a, n, w, g);
   adr_update<R, ND, NSM>(a, n, k, g, cnew);
   return true;
 }
@@ -440,6 +512,7 @@ inline int adr_grid(int nd, int nx, int ny, int nz, AdrArgs<R> &a) {
     return PRT_E_RANGE;
   }
   a.div_yz = make_fastdiv(a.Y * a.Z);
+  a.mad24 = (long long)a.Y * a.Z * (long long)sizeof(R) < (1LL << 23);
   a.div_z = make_fastdiv(a.Z);
   return PRT_OK;
 }
@@ -490,7 +563,15 @@ inline int adr_args_from_params(const prt_adr_params *p, int nx, int ny, int nz,
     a.mobile[j] = p->mobile[j];
     a.pinned[j] = p->pinned[j];
     a.fed[j] = (R)p->fed[j];
-    for (int q = 0; q < 4; ++q) a.dface[j][q] = (R)p->dface[j][q];
+    for (int q = 0; q < 4; ++q) {
+      // the step multiplies a closed face's zero difference by its
+      // coefficient, which stays zero only for a finite one
+      if (!(p->dface[j][q] >= 0) || !std::isfinite((R)p->dface[j][q])) {
+        set_error("prt_adr_step: face diffusion coefficients must be finite and non-negative");
+        return PRT_E_ARG;
+      }
+      a.dface[j][q] = (R)p->dface[j][q];
+    }
   }
   int bz = 64;
   while (bz < a.Z && bz < 256) bz <<= 1;
