@@ -172,6 +172,23 @@ def test_check_physics_matches_reference(case):
         assert any(kw in s for s in got)
 
 
+def test_grid_peclet_warning():
+    # 3D/tools/prtlb_3d.py:2320-2334: a Peclet number the grid cannot resolve
+    # is warned about at set-up, an ordinary one is not
+    import contextlib
+    import io
+    g = np.full((80, 8), prtlb.PORE, np.uint8)
+    v = np.zeros((3, 80, 8), np.float32)
+    v[0] = 1.0
+    ch = prtlb.Chemistry(names=["Ac"], inlet=[1.0], c_init=[0.0], d_rel=[1.0], immobile=[False],
+                         biotic_on=False)
+    for pe, warned in ((5000.0, True), (50.0, False)):
+        buf = io.StringIO()
+        with contextlib.redirect_stdout(buf):
+            adr_host_setup(g, v, pe, 0.0, 1, chem=ch)
+        assert ("grid Peclet" in buf.getvalue()) == warned, pe
+
+
 def test_host_errors():
     with pytest.raises(ValueError):
         prtlb.trt_minus(1.0 / 0.5)

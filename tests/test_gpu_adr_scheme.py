@@ -1,0 +1,208 @@
+"""The transport scheme on the GPU against answers that exist without it:
+the reference's own scheme checks (3D/tools/prtlb_3d.py:1859-2524,
+"THE SCHEME ITSELF"), run through prtlb.solve_adr with the same grids,
+chemistries, Peclet and Damkohler numbers and the same tolerances."""
+import numpy as np
+import pytest
+from scipy.special import erfcinv
+
+import prtlb
+from prtlb import Chemistry, default_chemistry
+
+pytestmark = pytest.mark.gpu
+
+PORE, SOLID = prtlb.PORE, prtlb.SOLID
+
+
+def _channel(nx, ny, walls=False):
+    g = np.full((nx, ny), PORE, np.uint8)
+    if walls:
+        g[:, 0] = g[:, -1] = SOLID
+    v = np.zeros((3, nx, ny), np.float32)
+    v[0] = 1.0
+    return g, v
+
+
+def _tracer():
+    return Chemistry(names=["Ac"], inlet=[1.0], c_init=[0.0], d_rel=[1.0],
+                     immobile=[False], biotic_on=False)
+
+
+def test_plug_flow_3d_and_maximum_principle():
+    # 3D/tools/prtlb_3d.py:1876-1890
+    g = np.full((40, 6, 6), PORE, np.uint8)
+    v = np.zeros((3, 40, 6, 6), np.float32)
+    v[0] = 1.0
+    out, _ = prtlb.solve_adr(g, v, pe=4000.0, da=1.0, n_t=3, n_species=1)
+    assert abs(float(out[-1, 0, -1].mean()) - np.exp(-1.0)) < 0.06
+    assert out.max() <= 1.0 + 1e-4
+
+
+@pytest.mark.parametrize("pe,tol", [(20.0, 0.10), (50.0, 0.08), (100.0, 0.08)])
+def test_front_width_is_the_erfc_width(pe, tol):
+    # 3D/tools/prtlb_3d.py:2266-2297: a step fed into an empty channel
+    # spreads as erfc((x - ut)/sqrt(4Dt))/2; its 90%-10% width is
+    # 2 erfcinv(0.2) sqrt(4Dt), compared without fitting anything
+    nx = 64
+    g, v = _channel(nx, 8)
+    i0 = {}
+    prtlb.solve_adr(g, v, pe=pe, da=0.0, n_t=2, n_species=1, steps=1, chem=_tracer(), info=i0)
+    nsteps = int(round((nx * 0.5) / i0["dt"]))
+    out, _ = prtlb.solve_adr(g, v, pe=pe, da=0.0, n_t=2, n_species=1, steps=nsteps, chem=_tracer())
+    prof = out[-1, 0].mean(1).astype(float)
+    xs = np.arange(nx) + 0.5
+    x90, x10 = np.interp([0.9, 0.1], prof[::-1], xs[::-1])
+    w = abs(x10 - x90)
+    w_exact = 2 * erfcinv(0.2) * np.sqrt(4 * (nx / pe) * nsteps * i0["dt"])
+    print("Pe", pe, "width", w, "exact", w_exact, flush=True)
+    assert abs(w - w_exact) / w_exact < tol
+
+
+def test_monotone_front_stays_monotone():
+    # 3D/tools/prtlb_3d.py:2299-2318: the limiter leaves no bump
+    g, v = _channel(80, 8)
+    out, _ = prtlb.solve_adr(g, v, pe=150.0, da=0.0, n_t=6, n_species=1, steps=400, chem=_tracer())
+    rises = [float(np.diff(out[i, 0].mean(1)).max()) for i in range(out.shape[0])]
+    assert max(rises) < 1e-6, rises
+
+
+@pytest.mark.parametrize("da", [20.0, 100.0, 400.0])
+def test_donor_plus_product_never_exceeds_the_feed(da):
+    # 3D/tools/prtlb_3d.py:2336-2355 ("biotic only", three chemicals, nearly
+    # zero-order donor kinetics, no acceptor demand, zero yield)
+    cm = default_chemistry(3)
+    cm.ks_donor = 1e-3
+    cm.alpha = 0.0
+    cm.inlet = np.array([1.0, 1.0, 0.0], np.float32)
+    cm.c_init = np.array([0.0, 1.0, 0.0], np.float32)
+    g = np.full((40, 8), PORE, np.uint8)
+    v = np.stack([np.ones((40, 8), np.float32), np.zeros((40, 8), np.float32),
+                  np.zeros((40, 8), np.float32)])
+    out, _ = prtlb.solve_adr(g, v, pe=30.0, da=da, n_t=3, n_species=3, chem=cm)
+    tot = float((out[-1, 0] + out[-1, 2]).max())
+    assert tot <= 1.02, tot
+
+
+def test_zero_acceptor_half_saturation_is_first_order():
+    # 3D/tools/prtlb_3d.py:2357-2366: Ks_A = 0 is first order in the
+    # acceptor, not 0/0
+    ck = default_chemistry(4)
+    ck.ks_acceptor = 0.0
+    g = np.full((30, 8), PORE, np.uint8)
+    v = np.stack([np.ones((30, 8), np.float32)] * 3)
+    out, _ = prtlb.solve_adr(g, v, pe=30.0, da=2.0, n_t=3, n_species=4, chem=ck)
+    assert np.isfinite(out).all()
+
+
+def test_stop_waits_for_the_slowest_chemical():
+    # 3D/tools/prtlb_3d.py:2399-2421: at the automatic stop the product and
+    # the biomass are within 1% of a run sixteen times as long
+    g, v = _channel(40, 8, walls=True)
+    chem = default_chemistry(4)
+    info = {}
+    stopped, _ = prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=3, n_species=4, chem=chem, info=info)
+    long_run, _ = prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=3, n_species=4, chem=chem,
+                                  steps=info["steps"] * 16)
+    pm = g == PORE
+    for j in (2, 3):
+        f_stop = float(stopped[-1, j][pm].mean())
+        f_long = float(long_run[-1, j][pm].mean())
+        print("chemical", j, f_stop, f_long, flush=True)
+        assert abs(f_stop - f_long) / max(abs(f_long), 1e-9) < 0.01
+
+
+def test_peclet_uses_the_axial_velocity():
+    # 3D/tools/prtlb_3d.py:2423-2436: a transverse component (faster in
+    # speed, same axial mean) does not slow the front down
+    g = np.full((40, 16), PORE, np.uint8)
+    v_ax = np.zeros((3, 40, 16), np.float32)
+    v_ax[0] = 1.0
+    v_tt = v_ax.copy()
+    v_tt[1] = 0.6
+    a, _ = prtlb.solve_adr(g, v_ax, pe=30.0, da=0.0, n_t=2, n_species=1, steps=200, chem=_tracer())
+    b, _ = prtlb.solve_adr(g, v_tt, pe=30.0, da=0.0, n_t=2, n_species=1, steps=200, chem=_tracer())
+
+    def reach(f):
+        return float(np.argmax(f[-1, 0].mean(1) < 0.5))
+
+    assert abs(reach(a) - reach(b)) <= 1, (reach(a), reach(b))
+
+
+def test_divergent_field_makes_no_solute():
+    # 3D/tools/prtlb_3d.py:2438-2454: a 10% compressible field; a chemical
+    # that is only transported never rises above its feed
+    g = np.full((24, 16), PORE, np.uint8)
+    g[:, 0] = g[:, -1] = SOLID
+    xs = np.arange(24)[:, None] * np.ones((1, 16))
+    v = np.zeros((3, 24, 16), np.float32)
+    v[0] = 1.0 + 0.10 * np.sin(2 * np.pi * xs / 24.0)
+    ch = Chemistry(names=["Ac", "A"], inlet=[1.0, 1.0], c_init=[0.0, 1.0], d_rel=[1.0, 1.0],
+                   immobile=[False, False], biotic_on=False)
+    out, _ = prtlb.solve_adr(g, v, pe=2.0, da=0.0, n_t=3, n_species=2, steps=40000, chem=ch)
+    pd = g == PORE
+    worst = max(float(out[:, 0][:, pd].max()), float(out[:, 1][:, pd].max()))
+    assert worst <= 1.0005, worst
+
+
+def test_abiotic_reaction_converts_donor_to_product():
+    # 3D/tools/prtlb_3d.py:2485-2524 ("abiotic only": the donor is the
+    # reactant and P the product, one mole for one; "none" is a pure loss)
+    g, v = _channel(60, 16, walls=True)
+    ma = g == PORE
+
+    def chem(product):
+        c = default_chemistry(4)
+        c.biotic_on = False
+        c.abiotic_index = 0
+        c.abiotic_product_index = 2 if product else None
+        c.abiotic_surface_only = False
+        c.c_init = np.array([0.0, 0.0, 0.0, 0.0], np.float32)
+        return c
+
+    prev = -1.0
+    for da_a in (0.5, 5.0):
+        c = chem(True)
+        out, _ = prtlb.solve_adr(g, v, pe=10.0, da=0.0, n_t=3, n_species=3, chem=c, da_abio=da_a)
+        ac = float(out[-1, 0][ma].mean())
+        p = float(out[-1, 2][ma].mean())
+        print("Da_abio", da_a, "Ac", ac, "P", p, flush=True)
+        assert p > prev and p > 0.05
+        assert abs(ac + p - 1.0) < 0.01
+        assert not prtlb.check_physics(out, g, chem=c, da_abio=da_a)
+        prev = p
+    out, _ = prtlb.solve_adr(g, v, pe=10.0, da=0.0, n_t=3, n_species=3, chem=chem(False), da_abio=5.0)
+    assert float(out[-1, 2][ma].max()) == 0.0
+    assert float(out[-1, 0][ma].mean()) < 0.5
+
+
+def test_check_physics_on_a_sound_run_and_planted_faults():
+    # 3D/tools/prtlb_3d.py:2368-2397: quiet on a sound four-chemical run,
+    # and each planted fault is named
+    g, v = _channel(40, 8, walls=True)
+    chem = default_chemistry(4)
+    op, _ = prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=4, n_species=4, chem=chem)
+    assert not prtlb.check_physics(op, g, chem=chem)
+    b = op.copy()
+    b[:, 2] = 5.0
+    assert any("P reaches" in c for c in prtlb.check_physics(b, g, chem=chem))
+    b = op.copy()
+    b[:, 1] = -0.5 * (g == PORE)
+    assert any("negative" in c for c in prtlb.check_physics(b, g, chem=chem))
+    b = op.copy()
+    b[:, 0] = 0.0
+    assert any("exactly zero" in c for c in prtlb.check_physics(b, g, chem=chem))
+    b = op.copy()
+    b[:, 3] = 500.0
+    assert any("runaway" in c for c in prtlb.check_physics(b, g, chem=chem))
+    xs = np.arange(40)[:, None] * np.ones((1, 8))
+    b = op.copy()
+    b[-1, 0] = np.where(g == PORE, 0.5 + 0.4 * ((-1.0) ** xs), 0.0)
+    assert any("checkerboard" in c for c in prtlb.check_physics(b, g, chem=chem))
+    zs = np.arange(8)[None, :] * np.ones((40, 1))
+    b = op.copy()
+    b[-1, 0] = np.where(g == PORE, 0.5 + 0.4 * ((-1.0) ** zs), 0.0)
+    assert not any("checkerboard" in c for c in prtlb.check_physics(b, g, chem=chem))
+    z = op.copy()
+    z[:, 2] = 0.0
+    assert any("never happened" in c for c in prtlb.check_physics(z, g, chem=chem, da_abio=3.78))
+    assert not any("never happened" in c for c in prtlb.check_physics(z, g, chem=chem, da_abio=0.0))
