@@ -75,6 +75,13 @@ size_t prt_adr_params_size(void);
 
 /* ------------------------------------------------------------------ Stokes */
 
+/* The lattice the kernels stream and collide with, copied to host arrays:
+ * c (3q int32, velocity i at c[3i .. 3i+2] in the reference's (x, y, z)
+ * axes; D2Q9 has its y component in slot 1 and 0 in slot 2), w (q doubles)
+ * and opp (q int32).  q = 9 follows 3D/tools/prtlb_2d.py:391-394, q = 19
+ * 3D/tools/prtlb_3d.py:370-375, order included.  Host only. */
+int prt_lbm_lattice(int q, int32_t *c, double *w, int32_t *opp);
+
 /* Elements of one population buffer of a dense (nx, ny, nz) grid (z-row
  * layout above), or -1 for a bad grid. */
 int64_t prt_lbm_pop_elems(int q, int storage, int nx, int ny, int nz);

@@ -381,6 +381,28 @@ extern "C" int64_t prt_lbm_partials_count(int q, int nx, int ny, int nz) {
   return (int64_t)(zb * g.Y * g.X);
 }
 
+template <int Q>
+static void lattice_copy(int32_t *c, double *w, int32_t *opp) {
+  for (int i = 0; i < Q; ++i) {
+    // kernels keep D2Q9 as (x, -, y) in their (x, y, z) slots
+    c[3 * i] = Lat<Q>::cx[i];
+    c[3 * i + 1] = Q == 9 ? Lat<Q>::cz[i] : Lat<Q>::cy[i];
+    c[3 * i + 2] = Q == 9 ? 0 : Lat<Q>::cz[i];
+    w[i] = weight<Q>(i);
+    opp[i] = Lat<Q>::opp[i];
+  }
+}
+
+extern "C" int prt_lbm_lattice(int q, int32_t *c, double *w, int32_t *opp) {
+  if ((q != 9 && q != 19) || !c || !w || !opp) {
+    set_error("prt_lbm_lattice: q must be 9 or 19 and the arrays non-null");
+    return PRT_E_ARG;
+  }
+  if (q == 9) lattice_copy<9>(c, w, opp);
+  else lattice_copy<19>(c, w, opp);
+  return PRT_OK;
+}
+
 extern "C" int64_t prt_lbm_pop_elems(int q, int storage, int nx, int ny, int nz) {
   Geom g;
   if (storage < 0 || storage > 2) {

@@ -85,6 +85,7 @@ _SIGS = {
     "prt_adr_params_size": (ctypes.c_size_t, []),
     "prt_lbm_fill_rest": (_I, [_I, _I, _I, _I, _I, _P, _P]),
     "prt_lbm_pop_elems": (_I64, [_I, _I, _I, _I, _I]),
+    "prt_lbm_lattice": (_I, [_I, _P, _P, _P]),
     "prt_lbm_step": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _D, _D, _D, _D, _I, _P, _P]),
     "prt_lbm_partials_count": (_I64, [_I, _I, _I, _I]),
     "prt_lbm_run": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _I, _D, _D, _D, _D, _I, _P, _P]),

@@ -172,6 +172,28 @@ def test_check_physics_matches_reference(case):
         assert any(kw in s for s in got)
 
 
+@pytest.mark.parametrize("q,nd", [(9, 2), (19, 3)])
+def test_lattice_tables(q, nd):
+    """The kernels' lattice: the identities 3D/tools/test_flow_solvers.py:51-73
+    checks (sum w = 1, sum w c = 0, sum w c c = I/3, opposites) to 1e-12, and
+    the reference's velocity order and weights exactly."""
+    c = np.zeros(3 * q, np.int32)
+    w = np.zeros(q, np.float64)
+    opp = np.zeros(q, np.int32)
+    _lib.call("prt_lbm_lattice", q, c.ctypes.data, w.ctypes.data, opp.ctypes.data)
+    c = c.reshape(q, 3)
+    assert not c[:, nd:].any()
+    c = c[:, :nd].astype(np.float64)
+    assert abs(w.sum() - 1.0) < 1e-12
+    assert np.abs(w @ c).max() < 1e-12
+    assert np.abs(np.einsum("i,ia,ib->ab", w, c, c) - np.eye(nd) / 3.0).max() < 1e-12
+    assert (c[opp] == -c).all() and (opp[opp] == np.arange(q)).all()
+    e, w_ref, opp_ref = O.lattice(nd)
+    assert np.array_equal(c, e) and np.array_equal(w, w_ref) and np.array_equal(opp, opp_ref)
+    with pytest.raises(_lib.PrtError):
+        _lib.call("prt_lbm_lattice", 15, c.ctypes.data, w.ctypes.data, opp.ctypes.data)
+
+
 def test_grid_peclet_warning():
     # 3D/tools/prtlb_3d.py:2320-2334: a Peclet number the grid cannot resolve
     # is warned about at set-up, an ordinary one is not
