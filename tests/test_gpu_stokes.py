@@ -161,6 +161,37 @@ def test_mass_conserved_full_size():
     assert np.isfinite(s.permeability(sux))
 
 
+def _duct3d(nx=16, H=14):
+    g = np.full((nx, H + 2, H + 2), prtlb.PORE, np.uint8)
+    g[:, 0, :] = g[:, -1, :] = prtlb.SOLID
+    g[:, :, 0] = g[:, :, -1] = prtlb.SOLID
+    return g
+
+
+def test_porous_run_and_default_iteration_counts():
+    """3D/tools/test_flow_solvers.py:284-304 and :372-386: a porous 2D run
+    has exactly zero velocity in solid and stays below Mach 0.05, a walled
+    3D duct has none in its walls, and the default iteration counts (1200 in
+    2D, 400 in 3D) are within 5% of the mean speed at eight times as many.
+    A seeded, inlet-connected disk pack of porosity 0.7 on 64 x 32 stands in
+    for the reference's random blob of the same size and porosity."""
+    g, _ = prtlb.keep_inlet_connected(prtlb.disk_pack((64, 32), 3.0, 0.7, seed=3))
+    v = prtlb.stokes_d2q9(g, nit=3000)
+    assert float(np.abs(v[:, g != prtlb.PORE]).max()) == 0.0
+    assert float(np.sqrt((v[:2] ** 2).sum(0)).max()) / np.sqrt(1.0 / 3.0) < 0.05
+    d = _duct3d()
+    v3 = prtlb.stokes_d3q19(d, nit=2000)
+    assert float(np.abs(v3[:, d != prtlb.PORE]).max()) == 0.0
+    for solver, geom, default in ((prtlb.stokes_d2q9, g, 1200), (prtlb.stokes_d3q19, d, 400)):
+        a = solver(geom, nit=default)
+        b = solver(geom, nit=default * 8)
+        m = geom == prtlb.PORE
+        sa = np.sqrt((a[:2] ** 2).sum(0))[m].mean()
+        sb = np.sqrt((b[:2] ** 2).sum(0))[m].mean()
+        print("default", default, "mean speed", sa, sb, flush=True)
+        assert abs(sa - sb) / max(sb, 1e-30) < 0.05
+
+
 def test_plane_flux_constant_and_solid_zero():
     g = np.full((16, 16, 16), prtlb.PORE, np.uint8)
     g[:, 0, :] = g[:, -1, :] = prtlb.SOLID
