@@ -206,3 +206,135 @@ def test_check_physics_on_a_sound_run_and_planted_faults():
     z[:, 2] = 0.0
     assert any("never happened" in c for c in prtlb.check_physics(z, g, chem=chem, da_abio=3.78))
     assert not any("never happened" in c for c in prtlb.check_physics(z, g, chem=chem, da_abio=0.0))
+
+
+# ---- the run's bookkeeping and the chemistry switches
+# (3D/tools/prtlb_3d.py:1899-2255), with the reference's Settings edits
+# written as the Chemistry attributes they set
+
+def test_snapshot_times_and_info():
+    # :1899-1915: n_t = 2 spans 0..1; 11 snapshots of a 200-step run are
+    # distinct; an explicit step count is recorded as not converged
+    g, v = _channel(40, 6)
+    _, tn = prtlb.solve_adr(g, v, pe=100.0, da=1.0, n_t=2, n_species=1)
+    assert tn[0] == 0.0 and abs(tn[-1] - 1.0) < 1e-6
+    info = {}
+    _, tn = prtlb.solve_adr(g, v, pe=100.0, da=1.0, n_t=11, n_species=1, steps=200, info=info)
+    assert len(set(tn.tolist())) == 11 and info["repeated_snapshots"] == 0
+    assert info["converged"] is False
+
+
+def test_default_chemistry_run():
+    # :1917-1994: the biomass survives at the inlet face; check_physics flags
+    # an all-NaN field and is quiet on this run; default chemistry equals
+    # chem=None bit for bit; the product never exceeds its donor
+    g, v = _channel(40, 8)
+    out, tb = prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=3, n_species=4)
+    assert float(out[-1, 3, 0].mean()) > 0.5 * 0.1
+    nan_field = np.full((3, 1, 20, 8), np.nan, np.float32)
+    assert any("not finite" in s for s in prtlb.check_physics(nan_field, np.full((20, 8), PORE, np.uint8)))
+    assert prtlb.check_physics(out, g) == []
+    base, tb = prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=4, n_species=4)
+    same, ts = prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=4, n_species=4, chem=default_chemistry(4))
+    assert np.array_equal(base, same) and np.array_equal(tb, ts)
+    assert float(base[:, 2].max()) <= 1.0 + 1e-4
+
+
+def test_per_chemical_diffusivity_is_used():
+    # :2000-2022: with no flow and no reaction a chemical given a tenth of
+    # the diffusivity spreads less far, the other one exactly as before
+    g = np.full((40, 8), PORE, np.uint8)
+    still = np.zeros((3, 40, 8), np.float32)
+    s = default_chemistry(2)
+    slow = default_chemistry(2)
+    slow.d_rel = np.array([1.0, 0.1], np.float32)
+    slow.d_biofilm_rel = slow.d_rel.copy()
+    a1, _ = prtlb.solve_adr(g, still, pe=1.0, da=1e-9, n_t=2, n_species=2, steps=3000, chem=s)
+    a2, _ = prtlb.solve_adr(g, still, pe=1.0, da=1e-9, n_t=2, n_species=2, steps=3000, chem=slow)
+    assert float(a2[-1, 1].mean()) < 0.6 * float(a1[-1, 1].mean())
+    assert abs(float(a1[-1, 0].mean()) - float(a2[-1, 0].mean())) < 1e-6
+    assert prtlb.check_physics(a2, g, chem=slow) == []
+
+
+def test_unfed_acceptor_only_decreases():
+    # :2024-2036: an acceptor placed in the sample (no feed) starts at 1 and
+    # then only ever decreases
+    g, v = _channel(40, 8)
+    c = default_chemistry(4)
+    c.inlet = np.array([1.0, np.nan, 0.0, np.nan], np.float32)
+    c.c_init = np.array([0.0, 1.0, 0.0, 0.1], np.float32)
+    pre, _ = prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=6, n_species=4, chem=c)
+    prof = [float(pre[t, 1][g == PORE].mean()) for t in range(6)]
+    assert abs(prof[0] - 1.0) < 1e-5
+    assert all(prof[i + 1] <= prof[i] + 1e-6 for i in range(5)), prof
+
+
+def test_biomass_mobility_kinetics_and_coupling_switches():
+    # :2038-2055, :2160-2187: planktonic biomass sits further downstream than
+    # sessile; a donor half-saturation constant and coupling the rate to the
+    # biomass each change the product
+    g, v = _channel(40, 8)
+    base, _ = prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=4, n_species=4)
+
+    def downstream(f):
+        return float(f[-1, 3, 20:].mean()) / max(float(f[-1, 3].mean()), 1e-30)
+
+    pla = default_chemistry(4)
+    pla.immobile = np.array([False, False, False, False])
+    pla.d_rel = np.array([1.0, 1.0, 1.0, 0.2], np.float32)
+    pla.d_biofilm_rel = np.array([1.0, 1.0, 1.0, 0.1], np.float32)
+    mob, _ = prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=4, n_species=4, chem=pla)
+    assert downstream(mob) > downstream(base)
+    mon = default_chemistry(4)
+    mon.ks_donor = 1.0e-3
+    o_m, _ = prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=4, n_species=4, chem=mon)
+    assert abs(float(o_m[-1, 2].mean()) - float(base[-1, 2].mean())) > 1e-4
+    cp = default_chemistry(4)
+    cp.biomass_coupled = True
+    o_c, _ = prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=4, n_species=4, chem=cp)
+    assert abs(float(o_c[-1, 2].mean()) - float(base[-1, 2].mean())) > 1e-4
+
+
+def test_abiotic_on_the_product_and_its_switches():
+    # :2057-2140 ("both": the abiotic reaction consumes the product)
+    g, v = _channel(40, 8)
+    base4, _ = prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=4, n_species=4)
+    ab, _ = prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=4, n_species=4, da_abio=5.0)
+    assert float(ab[-1, 2].mean()) < 0.6 * float(base4[-1, 2].mean())
+    non, _ = prtlb.solve_adr(g, v, pe=200.0, da=0.0, n_t=3, n_species=4, da_abio=8.0)
+    assert abs(float(non[-1, 0, -1].mean()) - 1.0) < 0.02
+    off, _ = prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=4, n_species=4, da_abio=0.0)
+    assert np.array_equal(off, base4)
+    # surface-only: no surface in an open channel, some along walls
+    s2 = default_chemistry(4)
+    s2.abiotic_surface_only = True
+    i_open = {}
+    prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=4, n_species=4, chem=s2, da_abio=5.0, info=i_open)
+    assert i_open["surface_voxels"] == 0.0
+    gw, vw = _channel(40, 8, walls=True)
+    b_w, _ = prtlb.solve_adr(gw, vw, pe=30.0, da=1.0, n_t=4, n_species=4, chem=s2)
+    i_wall = {}
+    a_w, _ = prtlb.solve_adr(gw, vw, pe=30.0, da=1.0, n_t=4, n_species=4, chem=s2, da_abio=5.0,
+                             info=i_wall)
+    assert float(a_w[-1, 2].mean()) < float(b_w[-1, 2].mean()) - 1e-4 and i_wall["surface_voxels"] > 0
+    with pytest.raises(ValueError, match="cannot happen"):
+        prtlb.solve_adr(gw, vw, pe=30.0, da=1.0, n_t=3, n_species=2, chem=default_chemistry(4), da_abio=5.0)
+
+
+def test_no_microbes_means_no_biomass():
+    # :2236-2252: with the biotic reaction off the biomass is exactly zero,
+    # the abiotic reaction still consumes the donor, a biotic run has biomass
+    g, v = _channel(40, 8)
+    c = default_chemistry(4)
+    c.biotic_on = False
+    c.abiotic_index = 0
+    c.abiotic_product_index = 2
+    c.c_init = np.array([0.0, 0.0, 0.0, 0.0], np.float32)
+    o_ab, _ = prtlb.solve_adr(g, v, pe=30.0, da=0.0, n_t=3, n_species=4, chem=c, da_abio=2.0)
+    assert float(np.abs(o_ab[:, 3]).max()) == 0.0
+    assert float(o_ab[-1, 0, -1].mean()) < 0.9
+    bio, _ = prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=3, n_species=4)
+    assert float(bio[:, 3].max()) > 0.05
+    info = {}
+    prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=3, n_species=2, info=info)
+    assert info.get("chemistry") is not None
