@@ -338,3 +338,33 @@ def test_no_microbes_means_no_biomass():
     info = {}
     prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=3, n_species=2, info=info)
     assert info.get("chemistry") is not None
+
+
+def test_biofilm_lining():
+    # :2189-2234: a biofilm lining is not solid, the attached biomass starts
+    # only there, a twentieth of the biofilm diffusivity changes the field,
+    # and with no biofilm voxel the biofilm settings change nothing
+    BF = 4
+    gb = np.full((40, 8), PORE, np.uint8)
+    gb[:, 0] = gb[:, -1] = BF
+    vb = np.zeros((3, 40, 8), np.float32)
+    vb[0] = 1.0
+
+    def lined(d_bf):
+        c = default_chemistry(4)
+        c.biofilm_code = BF
+        c.seed_in_biofilm = True
+        c.d_biofilm_rel = np.array(d_bf, np.float32)
+        return c
+
+    ob, _ = prtlb.solve_adr(gb, vb, pe=30.0, da=1.0, n_t=3, n_species=4, chem=lined([1.0, 1.0, 1.0, 0.0]))
+    lining = gb == BF
+    assert float(ob[-1, 0][lining].max()) > 0.1
+    assert float(ob[0, 3][lining].min()) > 0.0 and float(ob[0, 3][gb == PORE].max()) == 0.0
+    slow = lined([0.05, 0.05, 0.05, 0.0])
+    os_, _ = prtlb.solve_adr(gb, vb, pe=30.0, da=1.0, n_t=3, n_species=4, chem=slow)
+    assert abs(float(os_[-1, 0][lining].mean()) - float(ob[-1, 0][lining].mean())) > 1e-3
+    g, v = _channel(40, 8)
+    plain, _ = prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=3, n_species=4)
+    with_bf, _ = prtlb.solve_adr(g, v, pe=30.0, da=1.0, n_t=3, n_species=4, chem=slow.truncate(4))
+    assert np.array_equal(plain, with_bf)
