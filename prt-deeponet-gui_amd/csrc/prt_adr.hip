@@ -44,10 +44,6 @@ __global__ __launch_bounds__(256) void adr_list_kernel(AdrArgs<R> a, const int32
   adr_entry<R, ND, NSM, NTT>(a, pidx, word, face, k, npore, cnew);
 }
 
-// Pore lists at least this long stream their table with nontemporal loads:
-// the table (32 B per pore voxel in float32) then exceeds what the 256 MB
-// last-level cache keeps between steps next to the concentration arrays.
-constexpr uint32_t kNtTableEntries = 1u << 22;
 
 // Pore table of a list: the word and face velocities of every entry, or
 // kPoreWordNone (and zero faces) for an entry that is not a pore voxel.

@@ -222,6 +222,12 @@ using vec4 = R __attribute__((ext_vector_type(4)));
 template <typename R>
 using vec2 = R __attribute__((ext_vector_type(2)));
 
+// Pore lists (one list, or all members of a batch together) at least this
+// long stream their table with nontemporal loads: the table (32 B per pore
+// voxel in float32) then exceeds what the 256 MB last-level cache keeps
+// between steps next to the concentration arrays.
+constexpr uint32_t kNtTableEntries = 1u << 22;
+
 // Load of a pore-table entry: nontemporal (NTT) for lists too large to
 // stay in the last-level cache between steps, ordinary otherwise.
 template <bool NTT, typename T>

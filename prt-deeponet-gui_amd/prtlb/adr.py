@@ -388,25 +388,55 @@ class TransportSolver:
             self.latest = 1 - self.latest
         self.steps_done += steps
 
-    def pore_means(self):
-        """float(C[j][m].mean()) for every chemical, as NumPy forms it."""
-        if self.n_pore == 0:
-            return np.full(self.ns, np.nan)
+    def _pore_sums(self, out=None):
+        """Enqueue the float32 pore sums of the newest field, in NumPy's
+        pairwise order, into the device tensor out (ns floats; default the
+        solver's own); returns out."""
         c = self.current()
-        if self.precision == "fp64":
-            return (c[:, self.pore_idx].sum(1) / self.n_pore).cpu().numpy()
         if self._plan is None:
             plan, L, I, H = pairwise_plan(self.n_pore)
             self._plan = (torch.from_numpy(plan).to(self.dev), L, I, H)
             self.means_scratch = torch.empty(self.ns * (L + I), dtype=torch.float32,
                                              device=self.dev)
             self.means_out = torch.empty(self.ns, dtype=torch.float32, device=self.dev)
+        if out is None:
+            out = self.means_out
         plan, L, I, H = self._plan
         call("prt_pairwise_sum_f32", ptr(c), self.n, self.ns, ptr(self.pore_idx),
              self.n_pore, ptr(plan),
-             L, I, H, ptr(self.means_scratch), ptr(self.means_out), self._stream())
-        s = self.means_out.cpu().numpy()
-        return np.array([float(np.float32(np.float64(x) / np.float64(self.n_pore))) for x in s])
+             L, I, H, ptr(self.means_scratch), ptr(out), self._stream())
+        return out
+
+    def _means_of(self, sums):
+        """np.mean's float32 result from the float32 pore sums."""
+        return np.array([float(np.float32(np.float64(x) / np.float64(self.n_pore))) for x in sums])
+
+    def pore_means(self):
+        """float(C[j][m].mean()) for every chemical, as NumPy forms it."""
+        if self.n_pore == 0:
+            return np.full(self.ns, np.nan)
+        if self.precision == "fp64":
+            c = self.current()
+            return (c[:, self.pore_idx].sum(1) / self.n_pore).cpu().numpy()
+        return self._means_of(self._pore_sums().cpu().numpy())
+
+    def check_async(self):
+        """The pore sums of the newest field copied to pinned host memory
+        and a copy of the field, all enqueued without waiting: returns
+        (host sums, event marking their arrival, field copy).  float32 only."""
+        if self.n_pore == 0 or self.precision == "fp64":
+            raise PrtError("check_async: float32 pore-list solves only")
+        dev = torch.empty(self.ns, dtype=torch.float32, device=self.dev)
+        self._pore_sums(dev)
+        host = torch.empty(self.ns, dtype=torch.float32, pin_memory=True)
+        host.copy_(dev, non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(self.dev))
+        return host, ev, self.c[self.latest].clone()
+
+    def restore(self, state):
+        """Make a field copy from check_async the newest field again."""
+        self.c[self.latest].copy_(state)
 
     def snapshot(self):
         return self.current().clone()
@@ -464,6 +494,7 @@ class AdrRun:
         self.prev_change = np.zeros(ns)
         self.n = 0
         self.nxt = 1
+        self.pending = []                    # deferred checks (arrive(defer=True))
         self.finished = self.limit <= 0      # the reference's loop never runs
         self._record(0)
 
@@ -477,31 +508,66 @@ class AdrRun:
         nxt_check = (self.n // self.check + 1) * self.check
         return min(self.limit, self.nxt, nxt_check)
 
-    def arrive(self, n):
+    def arrive(self, n, defer=False):
         """The device field is now at step n (== next_target()); handle the
-        record / check there.  Sets self.finished when the run is over."""
+        record / check there.  Sets self.finished when the run is over.
+
+        defer: enqueue a check's pore means (and a copy of the field) and
+        decide it later, in resolve(), without waiting for the device here.
+        A run found settled at a deferred check is put back to that step, so
+        the result is the same as deciding at once."""
         self.n = n
         if n >= self.nxt:
             self._record(n)
             self.nxt = max(n + 1, int(np.ceil(n * self.ratio)))
         if n % self.check == 0:
-            cur = self.T.pore_means()
-            self.scale = np.maximum(self.scale, np.abs(cur))
-            change = float((np.abs(cur - self.prev) / self.scale).max())
-            step_ch = np.abs(cur - self.prev) / self.scale
-            with np.errstate(divide="ignore", invalid="ignore"):
-                r = np.where(self.prev_change > 0, step_ch / self.prev_change, 0.0)
-            rest = np.where(r < 1.0, step_ch * r / np.maximum(1.0 - r, 1e-12), np.inf)
-            rest = float(np.nanmax(np.where(np.isfinite(rest), rest, np.inf)))
-            if self.on_progress is not None:
-                self.on_progress(n, self.limit, float(cur[0]),
-                                 rest if np.isfinite(rest) else change)
-            if self.steps is None and n >= 10 * self.check and rest <= self.tol:
-                self.finished = True
+            if defer:
+                self.pending.append((n,) + self.T.check_async())
+            elif self._check(n, self.T.pore_means()):
                 return
-            self.prev, self.prev_change = cur, step_ch
         if n >= self.limit:
+            if self.pending:
+                self.resolve(block=True)
             self.finished = True
+
+    def resolve(self, block=False):
+        """Decide the deferred checks, oldest first; without block only those
+        whose means have reached the host.  A settled run is put back to the
+        step it settled at."""
+        while self.pending:
+            n, host, ev, state = self.pending[0]
+            if not block and not ev.query():
+                return
+            ev.synchronize()
+            self.pending.pop(0)
+            if self._check(n, self.T._means_of(host.numpy())):
+                self.pending.clear()
+                self.T.restore(state)
+                self.T.steps_done = n
+                self.n = n
+                while self.keep_steps and self.keep_steps[-1] > n:
+                    self.keep_steps.pop()
+                    self.keep.pop()
+                return
+
+    def _check(self, n, cur):
+        """The reference's convergence check at step n with pore means cur;
+        True (and finished) when the run stops here."""
+        self.scale = np.maximum(self.scale, np.abs(cur))
+        change = float((np.abs(cur - self.prev) / self.scale).max())
+        step_ch = np.abs(cur - self.prev) / self.scale
+        with np.errstate(divide="ignore", invalid="ignore"):
+            r = np.where(self.prev_change > 0, step_ch / self.prev_change, 0.0)
+        rest = np.where(r < 1.0, step_ch * r / np.maximum(1.0 - r, 1e-12), np.inf)
+        rest = float(np.nanmax(np.where(np.isfinite(rest), rest, np.inf)))
+        if self.on_progress is not None:
+            self.on_progress(n, self.limit, float(cur[0]),
+                             rest if np.isfinite(rest) else change)
+        if self.steps is None and n >= 10 * self.check and rest <= self.tol:
+            self.finished = True
+            return True
+        self.prev, self.prev_change = cur, step_ch
+        return False
 
     def result(self, info=None, da=0.0, da_abio=0.0, vel=None):
         """(out, t_norm) exactly as the reference assembles them, plus info."""

@@ -331,6 +331,11 @@ class AdrBatch:
     # the membership (the plan) is unchanged
     GRAPH_STEPS = 32
     use_graph = True
+    # convergence checks are decided without stopping the device: a member's
+    # check is read once its means reach the host (eager), and at the latest
+    # when it has more than check_lag undecided checks
+    eager = True
+    check_lag = 2
 
     def _launch(self, parity, steps):
         s = stream_ptr(self.dev)
@@ -394,9 +399,11 @@ class AdrBatch:
             n = m.run.n + todo
             m.T.steps_done = n
             if n == m.run.next_target():
-                m.run.arrive(n)
+                m.run.arrive(n, defer=True)
             else:
                 m.run.n = n
+            if not m.run.finished and (self.eager or len(m.run.pending) > self.check_lag):
+                m.run.resolve(block=len(m.run.pending) > self.check_lag)
             if m.run.finished:
                 done.append(m)
         return done

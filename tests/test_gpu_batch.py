@@ -193,6 +193,36 @@ def test_adr_batch_equals_single():
     assert np.array_equal(got["b"][0], ob) and np.array_equal(got["b"][1], tb)
 
 
+def test_adr_batch_deferred_checks_restore_the_stop_step():
+    """A member whose convergence checks are decided several checks after
+    the device passed them is put back to the step it settled at: the same
+    snapshots, t_norm and stop step as solve_adr."""
+    g1 = np.full((40, 8), prtlb.PORE, np.uint8)
+    g1[:, 0] = g1[:, -1] = prtlb.SOLID
+    v1 = np.zeros((3, 40, 8), np.float32)
+    v1[0] = 1.0
+    g2 = prtlb.disk_pack((40, 20), 3.0, 0.6, seed=5)
+    v2 = prtlb.stokes_d2q9(g2, nit=2000, force=1e-5, tol=1e-6)
+    ab = AdrBatch()
+    ab.eager = False
+    ab.check_lag = 3
+    ab.add("a", g1, v1, 30.0, 1.0, 3, n_species=4)                          # settles
+    ab.add("b", g2, v2, 8.0, 0.5, 4, n_species=1, steps=40000)
+    got = {}
+    while len(ab):
+        for m in ab.advance():
+            info = {}
+            got[m.key] = ab.finish(m, info) + (info,)
+    ia, ib = {}, {}
+    oa, ta = prtlb.solve_adr(g1, v1, 30.0, 1.0, 3, n_species=4, info=ia)
+    ob, tb = prtlb.solve_adr(g2, v2, 8.0, 0.5, 4, n_species=1, steps=40000, info=ib)
+    print("stop step", ia["steps"], flush=True)
+    assert ia["converged"] and ia["steps"] < 40000
+    assert np.array_equal(got["a"][0], oa) and np.array_equal(got["a"][1], ta)
+    assert got["a"][2]["steps"] == ia["steps"] and got["a"][2]["converged"]
+    assert np.array_equal(got["b"][0], ob) and np.array_equal(got["b"][1], tb)
+
+
 def test_server_result_same_alone_and_in_mix():
     specs = [dict(shape=(96, 96), radius=6.0, porosity=0.6, geom_seed=1),
              dict(shape=(40, 40, 40), radius=(3.0, 6.0), porosity=0.4, geom_seed=2),
