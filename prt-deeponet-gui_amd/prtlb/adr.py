@@ -391,8 +391,8 @@ class TransportSolver:
     def _pore_sums(self, out=None):
         """Enqueue the float32 pore sums of the newest field, in NumPy's
         pairwise order, into the device tensor out (ns floats; default the
-        solver's own); returns out."""
-        c = self.current()
+        solver's own); returns out.  A row kept in list order is summed in
+        place, through an identity index, instead of through a box copy."""
         if self._plan is None:
             plan, L, I, H = pairwise_plan(self.n_pore)
             self._plan = (torch.from_numpy(plan).to(self.dev), L, I, H)
@@ -402,7 +402,17 @@ class TransportSolver:
         if out is None:
             out = self.means_out
         plan, L, I, H = self._plan
-        call("prt_pairwise_sum_f32", ptr(c), self.n, self.ns, ptr(self.pore_idx),
+        if self._pore_list and self._immobile:
+            if getattr(self, "_ident", None) is None:
+                self._ident = torch.arange(self.n_pore, dtype=torch.int32, device=self.dev)
+            c = self.c[self.latest]
+            for j in range(self.ns):
+                idx = self._ident if j in self._immobile else self.pore_idx
+                call("prt_pairwise_sum_f32", ptr(c[j]), self.n, 1, ptr(idx), self.n_pore,
+                     ptr(plan), L, I, H, ptr(self.means_scratch), ptr(out[j:j + 1]),
+                     self._stream())
+            return out
+        call("prt_pairwise_sum_f32", ptr(self.current()), self.n, self.ns, ptr(self.pore_idx),
              self.n_pore, ptr(plan),
              L, I, H, ptr(self.means_scratch), ptr(out), self._stream())
         return out
@@ -420,10 +430,11 @@ class TransportSolver:
             return (c[:, self.pore_idx].sum(1) / self.n_pore).cpu().numpy()
         return self._means_of(self._pore_sums().cpu().numpy())
 
-    def check_async(self):
+    def check_async(self, keep_state=True):
         """The pore sums of the newest field copied to pinned host memory
-        and a copy of the field, all enqueued without waiting: returns
-        (host sums, event marking their arrival, field copy).  float32 only."""
+        and (keep_state) a copy of the field, all enqueued without waiting:
+        returns (host sums, event marking their arrival, field copy or
+        None).  float32 only."""
         if self.n_pore == 0 or self.precision == "fp64":
             raise PrtError("check_async: float32 pore-list solves only")
         dev = torch.empty(self.ns, dtype=torch.float32, device=self.dev)
@@ -432,7 +443,7 @@ class TransportSolver:
         host.copy_(dev, non_blocking=True)
         ev = torch.cuda.Event()
         ev.record(torch.cuda.current_stream(self.dev))
-        return host, ev, self.c[self.latest].clone()
+        return host, ev, (self.c[self.latest].clone() if keep_state else None)
 
     def restore(self, state):
         """Make a field copy from check_async the newest field again."""
@@ -522,7 +533,9 @@ class AdrRun:
             self.nxt = max(n + 1, int(np.ceil(n * self.ratio)))
         if n % self.check == 0:
             if defer:
-                self.pending.append((n,) + self.T.check_async())
+                # only a check that may stop the run needs the field kept
+                may_stop = self.steps is None and n >= 10 * self.check
+                self.pending.append((n,) + self.T.check_async(keep_state=may_stop))
             elif self._check(n, self.T.pore_means()):
                 return
         if n >= self.limit:
