@@ -51,31 +51,48 @@ struct PlanBuilder {
 // Leaf sums: 8 accumulators, pairwise combine, sequential tail; below 8
 // elements a plain sequential sum from zero.  Mirrors NumPy's float32
 // pairwise_sum leaf (numpy/_core/src/umath/loops_utils.h.src).
-__global__ void pw_leaf_kernel(const float *c, long long ld, int rows, const int32_t *idx, const int32_t *lstart,
-                               const int32_t *llen, long long nleaves, long long stride_vals, float *vals) {
+// Eight lanes per leaf, lane j carrying accumulator r[j] (elements j, j+8,
+// j+16, ... of the leaf, in that order), so the 128-element leaves run in
+// parallel; the lanes' sums then meet in NumPy's order
+// ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)) through shuffles, and lane 0 adds the
+// tail.  Float addition is commutative, so each pair's sum is the same bits
+// whichever lane forms it.
+__global__ __launch_bounds__(256) void pw_leaf_kernel(const float *c, long long ld, int rows, const int32_t *idx,
+                                                      const int32_t *lstart, const int32_t *llen, long long nleaves,
+                                                      long long stride_vals, float *vals) {
 #pragma clang fp contract(off)
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nleaves * rows) return;
-  const int row = (int)(t / nleaves);
-  const long long leaf = t - (long long)row * nleaves;
+  const int j = (int)(t & 7);
+  const long long g = t >> 3;  // (row, leaf) pair of this group of 8 lanes
+  const bool live = g < nleaves * rows;
+  int row = 0, s = 0, n = 0;
+  if (live) {
+    row = (int)(g / nleaves);
+    const long long leaf = g - (long long)row * nleaves;
+    s = lstart[leaf];
+    n = llen[leaf];
+  }
   const float *a = c + (size_t)row * ld;
-  const int s = lstart[leaf], n = llen[leaf];
+  float r = 0.f;
+  const int body = n - (n % 8);
+  if (n >= 8) {
+    r = a[idx[s + j]];
+    for (int i = 8 + j; i < body; i += 8) r += a[idx[s + i]];
+  }
+  // every lane of the wave takes part in the shuffles
+  float p = r + __shfl_down(r, 1, 8);
+  p = p + __shfl_down(p, 2, 8);
+  p = p + __shfl_down(p, 4, 8);
+  if (!live || j != 0) return;
   float res;
   if (n < 8) {
     res = 0.f;
     for (int i = 0; i < n; ++i) res += a[idx[s + i]];
   } else {
-    float r[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) r[j] = a[idx[s + j]];
-    int i = 8;
-    for (; i < n - (n % 8); i += 8)
-#pragma unroll
-      for (int j = 0; j < 8; ++j) r[j] += a[idx[s + i + j]];
-    res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
-    for (; i < n; ++i) res += a[idx[s + i]];
+    res = p;
+    for (int i = body; i < n; ++i) res += a[idx[s + i]];
   }
-  vals[(size_t)row * stride_vals + leaf] = res;
+  vals[(size_t)row * stride_vals + (g - (long long)row * nleaves)] = res;
 }
 
 __global__ __launch_bounds__(1024) void pw_combine_kernel(const int32_t *left, const int32_t *right,
@@ -90,12 +107,19 @@ __global__ __launch_bounds__(1024) void pw_combine_kernel(const int32_t *left, c
       v[nleaves + k] = v[left[k]] + v[right[k]];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    // the buffer sums, left to right from zero
-    float acc = 0.f;
-    for (int k = 0; k < nchunks; ++k) acc = acc + v[roots[k]];
-    out[blockIdx.x] = acc;
+  // the buffer sums, left to right from zero: the workgroup stages 1024 of
+  // them at a time in LDS and one lane runs the chain from there
+  __shared__ float staged[1024];
+  float acc = 0.f;
+  for (int k0 = 0; k0 < nchunks; k0 += 1024) {
+    const int m = nchunks - k0 < 1024 ? nchunks - k0 : 1024;
+    if ((int)threadIdx.x < m) staged[threadIdx.x] = v[roots[k0 + threadIdx.x]];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int k = 0; k < m; ++k) acc = acc + staged[k];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) out[blockIdx.x] = acc;
 }
 
 }  // namespace prt
@@ -166,7 +190,7 @@ extern "C" int prt_pairwise_sum_f32(const float *c, int64_t ld, int rows, const 
   const int32_t *ls = plan, *ll = plan + L, *lf = plan + 2 * L, *rt = plan + 2 * L + I, *lo = plan + 2 * L + 2 * I;
   const int32_t *roots = lo + n_levels + 1;
   const int nchunks = (int)((n + 8191) / 8192);
-  const long long work = L * rows;
+  const long long work = 8 * L * rows;  // eight lanes per leaf
   pw_leaf_kernel<<<(unsigned)((work + 255) / 256), 256, 0, st>>>(c, ld, rows, idx, ls, ll, L, stride, scratch);
   int rc = launch_status("prt_pairwise_sum_f32 leaf launch");
   if (rc) return rc;
