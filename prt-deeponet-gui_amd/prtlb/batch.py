@@ -304,11 +304,22 @@ class AdrBatch:
         return sum(m.bytes for m in self.members.values())
 
     def add(self, key, mat, vel, pe, da, n_t, n_species=2, steps=None, tol=2e-3,
-            max_steps=100000, ratio=1.12, chem=None, da_abio=0.0, sorption=None):
+            max_steps=100000, ratio=1.12, chem=None, da_abio=0.0, sorption=None, host=None):
+        """host: the member's host set-up made beforehand with
+        adr.adr_host_setup(mat, vel, pe, da, n_species, chem, da_abio, "fp32",
+        sorption) (for instance on worker threads while the device runs);
+        mat, vel, pe, da, n_species, chem, da_abio and sorption must be the
+        ones it was made with."""
         if key in self.members:
             raise KeyError("member %r already present" % (key,))
-        T = TransportSolver(mat, vel, pe, da, n_species, chem, da_abio, precision="fp32",
-                            sorption=sorption, device=self.dev)
+        if host is not None:
+            if host.precision != "fp32":
+                raise ValueError("a batch member's host set-up must be float32")
+            T = host
+            T._setup_device(self.dev)
+        else:
+            T = TransportSolver(mat, vel, pe, da, n_species, chem, da_abio, precision="fp32",
+                                sorption=sorption, device=self.dev)
         m = _Seg()
         m.key, m.T = key, T
         m.phase = self.parity & 1

@@ -13,12 +13,13 @@ Two executions of the same campaign:
                   one AdrBatch, members admitted under an HBM budget
 """
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
 
 from . import chemistry as chemmod
-from .adr import solve_adr
+from .adr import adr_host_setup, solve_adr
 from .batch import AdrBatch, LbmBatch, adr_footprint, lbm_footprint
 from .geometry import PORE, disk_pack, sphere_pack
 from .stokes import stokes_d2q9, stokes_d3q19
@@ -204,9 +205,19 @@ def run_batched(specs, geoms, storage="fp32", tau=0.8, tol=1e-6, nit=30000,
                     progress("flow %s done (%d its)" % (key, m.conv.iters))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    # ---- transport: one batch
+    # ---- transport: one batch; the members' host set-ups (NumPy over each
+    # whole grid) run on worker threads, in order, while members join
     ab = AdrBatch()
     queue = list(specs)
+    pool = ThreadPoolExecutor(max_workers=8)
+
+    def prep(sp):
+        kw = _adr_kw(sp, kinetics)
+        return adr_host_setup(geoms[sp["key"]], velocities[sp["key"]], sp["pe"], kw["da"],
+                              kw["n_species"], kw["chem"], kw.get("da_abio", 0.0), "fp32",
+                              kw.get("sorption"))
+
+    hosts = {sp["key"]: pool.submit(prep, sp) for sp in queue}
     while queue or len(ab):
         while queue:
             sp = queue[0]
@@ -216,7 +227,7 @@ def run_batched(specs, geoms, storage="fp32", tau=0.8, tol=1e-6, nit=30000,
                 break
             queue.pop(0)
             ab.add(sp["key"], g, velocities[sp["key"]], sp["pe"], kw.pop("da"), n_t,
-                   steps=adr_steps, max_steps=max_steps, **kw)
+                   steps=adr_steps, max_steps=max_steps, host=hosts.pop(sp["key"]).result(), **kw)
         finished = ab.advance()
         if progress and time.perf_counter() - t_note > 20.0:
             t_note = time.perf_counter()
@@ -230,6 +241,7 @@ def run_batched(specs, geoms, storage="fp32", tau=0.8, tol=1e-6, nit=30000,
                      out_last=out[-1])
             if progress:
                 progress("transport %s done (%d steps)" % (m.key, ainfo["steps"]))
+    pool.shutdown()
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     return res, dict(total=t2 - t0, flow=t1 - t0, transport=t2 - t1)
