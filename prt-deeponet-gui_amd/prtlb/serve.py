@@ -17,10 +17,12 @@ per-process function (3D/tools/prtlb_3d.py:348, :781).
 """
 import time
 from collections import deque
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
 from . import campaign as C
+from .adr import adr_host_setup
 from .batch import AdrBatch, LbmBatch, adr_footprint, lbm_footprint
 from .geometry import PORE
 
@@ -51,6 +53,9 @@ class Server:
         self.n_t, self.adr_steps, self.max_steps = n_t, adr_steps, max_steps
         self.lb = {9: LbmBatch(9, storage, device=device), 19: LbmBatch(19, storage, device=device)}
         self.ab = AdrBatch(device=device)
+        # transport host set-ups (NumPy over the whole grid) run here, off
+        # the loop that keeps the device busy
+        self.pool = ThreadPoolExecutor(max_workers=4)
         self.waiting = deque()
         self.done = []
         self.reserved = {}
@@ -101,6 +106,10 @@ class Server:
                     continue
                 req.permeability = info["permeability"]
                 req.velocity = v
+                kw = C._adr_kw(dict(da=req.da), req.kinetics)
+                req.prep = self.pool.submit(adr_host_setup, req.g, v, req.pe, kw["da"],
+                                            kw["n_species"], kw["chem"], kw.get("da_abio", 0.0),
+                                            "fp32", kw.get("sorption"))
                 self.ready.append(req)
         self._admit_transport()
         if len(self.ab):
@@ -121,10 +130,15 @@ class Server:
             held = sum(m.T.n_pore for m in self.ab.members.values())
             if self.transport_cap and len(self.ab) and held + n_pore > self.transport_cap:
                 break
+            # a set-up still on its worker thread is waited for only when the
+            # device has nothing else to run
+            if not req.prep.done() and (len(self.ab) or any(len(b) for b in self.lb.values())):
+                break
             self.ready.popleft()
             kw = C._adr_kw(dict(da=req.da), req.kinetics)
             am = self.ab.add(req.key, req.g, req.velocity, req.pe, kw.pop("da"), self.n_t,
-                             steps=self.adr_steps, max_steps=self.max_steps, **kw)
+                             steps=self.adr_steps, max_steps=self.max_steps, host=req.prep.result(),
+                             **kw)
             am.req = req
 
     def idle(self):
