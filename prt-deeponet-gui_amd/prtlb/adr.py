@@ -638,10 +638,15 @@ def solve_adr(mat, vel, pe, da, n_t, n_species=2, steps=None, tol=2e-3,
                         precision=precision, sorption=sorption, device=device)
     run = AdrRun(T, n_t, steps=steps, tol=tol, max_steps=max_steps, ratio=ratio,
                  on_progress=on_progress)
+    # float32 checks are decided without waiting for the device: at most two
+    # stay undecided, and a run found settled late is rolled back (AdrRun)
+    defer = T.precision == "fp32" and T.n_pore > 0
     while not run.finished:
         target = run.next_target()
         T.step_n(target - run.n)
-        run.arrive(target)
+        run.arrive(target, defer=defer)
+        if defer and not run.finished:
+            run.resolve(block=len(run.pending) > 2)
     out, t_norm = run.result(info, da, da_abio, vel)
     if return_state:
         return out, t_norm, T
