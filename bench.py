@@ -493,7 +493,7 @@ def main():
                     help="only the flow-feature maps (harmonic pressure, MIS, UPRM, dw2)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: the library has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import numpy as np

@@ -4,8 +4,8 @@ TEST INFRASTRUCTURE ONLY: this module is the checker for the HIP solver in
 ``prt-deeponet-gui_amd/prtlb``.  Product code never imports it.
 
 Every function cites the reference file:line whose numerics it restates
-(reference = shahram444/PRT-DeepONet-GUI).  The layout here is this build's
-own: populations are held flattened as (Q, N) and streamed by gathering
+(reference = shahram444/PRT-DeepONet-GUI).  The layout here is the
+oracle's own: populations are held flattened as (Q, N) and streamed by gathering
 through a precomputed periodic pull table, and the transport stencil gathers
 through clamped (edge-replicated) neighbour index tables -- the same
 addressing the GPU kernels use -- instead of the reference's np.roll / np.pad

@@ -221,7 +221,7 @@ static int make_geom(int q, int nx, int ny, int nz, Geom &g, int flags = 0) {
   // every population element index (z-row layout, the widest row pitch of
   // the three storages) stays below 2^31
   if ((long long)g.X * g.Y * q * pop_row_pitch(g.Z, 2) >= (1LL << 31) || g.X > 65535) {
-    set_error("grid of %lld nodes exceeds the 32-bit node index of this build", g.N);
+    set_error("grid of %lld nodes exceeds the 32-bit node index", g.N);
     return PRT_E_RANGE;
   }
   // block shape: (bz, by) threads, npt nodes per thread along y.  Default:

@@ -5,7 +5,7 @@ raised) so the peak speed sits at Mach 0.02, the real Stokes solve, the
 axial pore mean speed; per (rock, Pe, Da): a transport solve.  Restates the
 caller logic of 3D/tools/build_dataset_3d.py:685-726 and
 3D/tools/build_dataset_2d.py:715-759 (the HDF5 writing is replaced by
-returned arrays; h5py is not part of this build).
+returned arrays; h5py is not a dependency here).
 
 Two executions of the same campaign:
   run_sequential  one solve at a time, stokes_d2q9/3q19 + solve_adr

@@ -2,7 +2,7 @@
 
 SOLID / WALL / PORE and the three helpers keep the reference's names and
 behaviour (3D/tools/prtlb_3d.py:478, :530-614).  The disk and sphere packs
-are this build's own seeded generators: the reference's dataset generators
+are this package's own seeded generators: the reference's dataset generators
 threshold Gaussian fields, so overlapping disks/spheres with periodic
 minimum-image distances are defined here, once, for every workload.
 """
