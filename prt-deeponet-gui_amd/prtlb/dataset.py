@@ -16,10 +16,12 @@ build, so the container is an .npz archive whose keys are the HDF5 paths
 "<path>.attrs.<name>" arrays; numpy.load(..., allow_pickle=False) reads it.
 """
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
 from . import campaign as C
+from .adr import adr_host_setup
 from .batch import AdrBatch, LbmBatch
 from .geometry import edt_device, geodesic_device, sphere_pack
 from .physics import check_physics_device
@@ -85,10 +87,20 @@ def build_dataset(n_geom=4, shape=(48, 48, 48), n_sets=2, n_times=5, porosity=(0
             d = float(np.exp(rng.uniform(np.log(da[0]), np.log(da[1]))))
             draws.append((i, p, d))
     ab = AdrBatch()
-    for k, (i, p, d) in enumerate(draws):
+
+    def prep(draw):
+        i, p, d = draw
         kw = C._adr_kw(dict(da=d), kinetics)
-        ab.add(k, mats[i], vel[i], p, kw.pop("da"), n_times, steps=adr_steps, max_steps=max_steps,
-               **kw)
+        return adr_host_setup(mats[i], vel[i], p, kw["da"], kw["n_species"], kw["chem"],
+                              kw.get("da_abio", 0.0), "fp32", kw.get("sorption"))
+
+    # the members' host set-ups run on worker threads while earlier ones join
+    with ThreadPoolExecutor(max_workers=8) as pool:
+        hosts = [pool.submit(prep, dr) for dr in draws]
+        for k, (i, p, d) in enumerate(draws):
+            kw = C._adr_kw(dict(da=d), kinetics)
+            ab.add(k, mats[i], vel[i], p, kw.pop("da"), n_times, steps=adr_steps,
+                   max_steps=max_steps, host=hosts[k].result(), **kw)
     results = {}
     while len(ab):
         for m in ab.advance():
