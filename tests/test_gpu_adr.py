@@ -195,3 +195,31 @@ def test_graph_replays_match_eager():
                                            abiotic_index=0, abiotic_surface_only=True),
                           da_abio=1.0)
     assert np.array_equal(out1, outo)
+
+
+@pytest.mark.parametrize("n", [1, 7, 8, 9, 127, 128, 129, 1000, 8192, 8193, 100003, 9_000_017])
+def test_device_pairwise_sum_equals_numpy(n):
+    """prt_pairwise_sum_f32 against np.add.reduce on float32 values, bit for
+    bit, over two rows gathered through an index: short leaves, ragged
+    leaf tails, one and many 8192-element buffers, and more than 1024 of
+    them (the chain then runs over several staged tiles)."""
+    import torch
+    from prtlb import _lib
+    from prtlb.device import ptr, stream_ptr
+    rng = np.random.default_rng(n)
+    ld = n + 5
+    rows = (rng.random((2, ld)) * rng.choice([1e-3, 1.0, 1e3], (2, ld))).astype(np.float32)
+    idx = np.sort(rng.choice(ld, n, replace=False)).astype(np.int32)
+    plan, L, I, H = _lib.pairwise_plan(n)
+    dev = torch.device("cuda:0")
+    c = torch.from_numpy(rows).to(dev)
+    ti = torch.from_numpy(idx).to(dev)
+    tp = torch.from_numpy(plan).to(dev)
+    scratch = torch.empty(2 * (L + I), dtype=torch.float32, device=dev)
+    out = torch.empty(2, dtype=torch.float32, device=dev)
+    print("case n", n, flush=True)
+    _lib.call("prt_pairwise_sum_f32", ptr(c), ld, 2, ptr(ti), n, ptr(tp), L, I, H, ptr(scratch),
+              ptr(out), stream_ptr(dev))
+    got = out.cpu().numpy()
+    for r in range(2):
+        assert got[r] == np.add.reduce(rows[r][idx]), (n, r)
