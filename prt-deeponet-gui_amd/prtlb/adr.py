@@ -450,7 +450,11 @@ class TransportSolver:
         self.c[self.latest].copy_(state)
 
     def snapshot(self):
-        return self.current().clone()
+        """A copy of the newest field in box layout (one copy: the layout
+        change already makes one when immobile rows are in list order)."""
+        c = self.c[self.latest]
+        box = self._to_box(c)
+        return box if box is not c else c.clone()
 
     def outlet_concentration(self, vel=None):
         """Build-defined outlet concentration (FP64) of the current field:
