@@ -217,31 +217,33 @@ def run_batched(specs, geoms, storage="fp32", tau=0.8, tol=1e-6, nit=30000,
                               kw["n_species"], kw["chem"], kw.get("da_abio", 0.0), "fp32",
                               kw.get("sorption"))
 
-    hosts = {sp["key"]: pool.submit(prep, sp) for sp in queue}
-    while queue or len(ab):
-        while queue:
-            sp = queue[0]
-            g = geoms[sp["key"]]
-            kw = _adr_kw(sp, kinetics)
-            if len(ab) and ab.bytes() + adr_footprint(g.shape, kw["n_species"], g.ndim) > hbm_budget:
-                break
-            queue.pop(0)
-            ab.add(sp["key"], g, velocities[sp["key"]], sp["pe"], kw.pop("da"), n_t,
-                   steps=adr_steps, max_steps=max_steps, host=hosts.pop(sp["key"]).result(), **kw)
-        finished = ab.advance()
-        if progress and time.perf_counter() - t_note > 20.0:
-            t_note = time.perf_counter()
-            progress("transport batch: %d members running, %d waiting" % (len(ab), len(queue)))
-        for m in finished:
-            ainfo = {}
-            out, tn = ab.finish(m, ainfo)
-            r = res[m.key]
-            r.update(adr_steps=ainfo["steps"], converged=ainfo["converged"],
-                     outlet=np.asarray(ainfo["outlet_concentration"]).tolist(),
-                     out_last=out[-1])
-            if progress:
-                progress("transport %s done (%d steps)" % (m.key, ainfo["steps"]))
-    pool.shutdown()
+    try:
+        hosts = {sp["key"]: pool.submit(prep, sp) for sp in queue}
+        while queue or len(ab):
+            while queue:
+                sp = queue[0]
+                g = geoms[sp["key"]]
+                kw = _adr_kw(sp, kinetics)
+                if len(ab) and ab.bytes() + adr_footprint(g.shape, kw["n_species"], g.ndim) > hbm_budget:
+                    break
+                queue.pop(0)
+                ab.add(sp["key"], g, velocities[sp["key"]], sp["pe"], kw.pop("da"), n_t,
+                       steps=adr_steps, max_steps=max_steps, host=hosts.pop(sp["key"]).result(), **kw)
+            finished = ab.advance()
+            if progress and time.perf_counter() - t_note > 20.0:
+                t_note = time.perf_counter()
+                progress("transport batch: %d members running, %d waiting" % (len(ab), len(queue)))
+            for m in finished:
+                ainfo = {}
+                out, tn = ab.finish(m, ainfo)
+                r = res[m.key]
+                r.update(adr_steps=ainfo["steps"], converged=ainfo["converged"],
+                         outlet=np.asarray(ainfo["outlet_concentration"]).tolist(),
+                         out_last=out[-1])
+                if progress:
+                    progress("transport %s done (%d steps)" % (m.key, ainfo["steps"]))
+    finally:
+        pool.shutdown(wait=False, cancel_futures=True)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     return res, dict(total=t2 - t0, flow=t1 - t0, transport=t2 - t1)
