@@ -18,8 +18,14 @@ Pinning (how each function is shown to be right, independently of itself):
                          stays exactly zero, uniform field invariant.
 * geometry helpers    -- golden vectors from the reference (scipy.ndimage).
 * ``check_physics``   -- golden complaint lists from the reference.
-* ``permeability``    -- slit closed form k = (2H^3+H)/(24(H+2)).
-* ``outlet_concentration`` -- plug flow exp(-Da).
+* ``stokes_c.stokes`` -- the same iteration in C (oracle/stokes_c.c), for
+                         full-size grids: bit-identical to ``stokes`` and
+                         held to the goldens (tests/test_oracle_c.py).
+* ``permeability``    -- slit closed form k = (2H^3+H)/(24(H+2)) to 1e-9 for
+                         H = 4..20 on the converged oracle flow
+                         (tests/test_oracle_c.py).
+* ``outlet_concentration`` -- plug flow exp(-Da) within 0.05 on a walled
+                         channel (tests/test_oracle_c.py).
 * ``sorption_reversible`` -- parity unpinned against the reference (the
                          reference has no reversible sorption); pinned by the
                          well-mixed closed form C(t)=Cinf+(C0-Cinf)e^{-(ka+kd)t}.

@@ -226,12 +226,6 @@ def permeability(v, g, tau_lb, force):
     return nu * float(v[0].sum() / v[0].size) / float(force)
 
 
-def plane_flux(v):
-    """Sum of u_x over each x-plane, FP64 (test_flow_solvers.py:342-350)."""
-    v = np.asarray(v, np.float64)
-    return v[0].reshape(v.shape[1], -1).sum(1)
-
-
 def slit_permeability(h):
     """Closed form of `permeability` for a slit of h fluid rows between two
     solid rows in a periodic (h+2)-wide box: the node-sampled Poiseuille

@@ -42,9 +42,17 @@ def _draw_rock(shape, phi, rng, radius, max_tries=50):
 def build_dataset(n_geom=4, shape=(48, 48, 48), n_sets=2, n_times=5, porosity=(0.3, 0.5),
                   radius=(3.0, 6.0), pe=(0.1, 100.0), da=(0.01, 10.0),
                   kinetics="sorption_irreversible", seed=0, tau=0.8, tol=1e-6, nit=30000,
-                  adr_steps=None, max_steps=100000, storage="fp32", d_ratio=1e-3, progress=None):
+                  adr_steps=None, max_steps=100000, storage="fp32", d_ratio=1e-3, progress=None,
+                  keep_complaints=False):
     """Run a campaign of n_geom rocks x n_sets transport runs; returns the
     dataset as a dict of arrays keyed like the reference's HDF5 file.
+
+    A run that check_physics complains about is printed as the reference
+    prints it, and then the campaign stops with SystemExit instead of
+    returning a dataset that looks like a good one
+    (3D/tools/build_dataset_3d.py:843-860).  keep_complaints=True returns
+    it anyway with the complaints under "_complaints", for inspection;
+    write_dataset still refuses it.
 
     samples/delta_P is the force giving each run's Peclet on its rock
     (pressure_for_peclet) with the sample length as reference length and
@@ -129,8 +137,15 @@ def build_dataset(n_geom=4, shape=(48, 48, 48), n_sets=2, n_times=5, porosity=(0
         u_ax = C.axial_mean(vel[i], mats[i])
         dP[k] = C.pressure_for_peclet(force[i], u_ax, p, shape[0], (tau - 0.5) / 3.0 * d_ratio)
         bad = check_physics_device(out, mats[i], chem=chem, da=kwk["da"], da_abio=kwk.get("da_abio"))
+        for msg in bad:
+            print("     PHYSICS run %d (Pe=%.2f Da_bio=%.2f Da_abio=%.2f): %s"
+                  % (k, p, kwk["da"], kwk.get("da_abio", 0.0), msg), flush=True)
         if bad:
             complaints[k] = bad
+    n_bad = sum(len(v) for v in complaints.values())
+    if n_bad and not keep_complaints:
+        raise SystemExit("%d physics complaint(s): this file is not usable as training data, "
+                         "and writing it would hide that." % n_bad)
     ds = write_layout(np.stack(mats), gdf, edt, conc, vels, gi, par, tn_all, settled, dP,
                       species=list(chem.names)[:ns])
     # not written: the check_physics complaints per run, and the float32
@@ -176,6 +191,15 @@ def write_layout(mat, gdf, edt, conc, vel, gi, par, tn, settled, dP, species, bi
 
 
 def write_dataset(path, ds):
-    """Write the layout dict to an .npz archive (no pickled objects)."""
+    """Write the layout dict to an .npz archive (no pickled objects).
+
+    Refuses (SystemExit, as 3D/tools/build_dataset_2d.py:889-894) a dataset
+    whose runs drew check_physics complaints: the file would look exactly
+    like a good one."""
+    bad = ds.get("_complaints") or {}
+    n_bad = sum(len(v) for v in bad.values())
+    if n_bad:
+        raise SystemExit("%d physics complaint(s) in %d run(s): this file is not usable as "
+                         "training data, so it has NOT been written." % (n_bad, len(bad)))
     np.savez(path, **{k: v for k, v in ds.items() if not k.startswith("_")})
     return path

@@ -280,3 +280,25 @@ def test_handle_api_host_side():
         _lib.call("prt_lbm_solver_create", 19, 0, 4, 4, 4, p, 10, p + 8, p + 16, p + 24, p + 32,
                   0.5, 3.0 / 16.0, 2e-6, 1024.0, 0, None, ctypes.byref(h))     # tau = 1/2
     assert L.prt_lbm_launch_blocks(19, 8, 8, 130, 0) > 0
+
+
+def test_write_dataset_refuses_complaints(tmp_path):
+    """A dataset whose runs drew check_physics complaints is not written
+    (3D/tools/build_dataset_2d.py:889-894); a clean one is."""
+    from prtlb import dataset as D
+    shape = (6, 5, 4)
+    mat = np.full((1,) + shape, prtlb.PORE, np.uint8)
+    conc = np.zeros((2, 3, 1) + shape, np.float32)
+    ds = D.write_layout(mat, np.zeros((1,) + shape, np.float32), np.zeros((1,) + shape, np.float32),
+                        conc, np.zeros((2, 3) + shape, np.float32), np.zeros(2, np.int32),
+                        np.zeros((2, 3), np.float32), np.zeros((2, 3), np.float32),
+                        np.ones(2, bool), np.zeros(2, np.float32), species=["C"])
+    ds["_complaints"] = {1: ["C: the field went non-finite"]}
+    path = tmp_path / "bad.npz"
+    with pytest.raises(SystemExit, match="NOT been written"):
+        D.write_dataset(str(path), ds)
+    assert not path.exists()
+    ds["_complaints"] = {}
+    D.write_dataset(str(tmp_path / "good.npz"), ds)
+    z = np.load(str(tmp_path / "good.npz"), allow_pickle=False)
+    assert z["geom/material"].shape == (1,) + shape

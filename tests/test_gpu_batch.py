@@ -285,3 +285,26 @@ def test_dataset_layout_and_content():
     D.write_dataset(path, ds)
     z = np.load(path, allow_pickle=False)
     assert "samples/conc" in z.files and z["samples/settled"].dtype == bool
+
+
+def test_dataset_stops_on_physics_complaint(monkeypatch, capsys):
+    """A planted complaint (the host's check returns one for run 1) stops
+    the campaign before a dataset exists, with the reference's PHYSICS line
+    (3D/tools/build_dataset_3d.py:843-860)."""
+    from prtlb import dataset as D
+    real = D.check_physics_device
+    calls = []
+
+    def planted(out, mat, **kw):
+        calls.append(1)
+        bad = real(out, mat, **kw)
+        return bad + (["C: planted complaint"] if len(calls) == 2 else [])
+
+    monkeypatch.setattr(D, "check_physics_device", planted)
+    with pytest.raises(SystemExit, match="1 physics complaint"):
+        D.build_dataset(n_geom=1, shape=(20, 16, 16), n_sets=2, n_times=3, radius=(2.0, 4.0),
+                        adr_steps=100, seed=4)
+    assert "PHYSICS run 1" in capsys.readouterr().out
+    ds = D.build_dataset(n_geom=1, shape=(20, 16, 16), n_sets=2, n_times=3, radius=(2.0, 4.0),
+                         adr_steps=100, seed=4, keep_complaints=True)
+    assert list(ds["_complaints"]) == [1]

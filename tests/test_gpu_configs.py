@@ -1,0 +1,186 @@
+"""The benchmark workloads end to end on the GPU, field by field against the
+oracle at their full sizes.
+
+Config 1 (the CPU-runnable case): 128^2 disk pack (r 6, porosity ~0.6,
+seed 0), D2Q9 TRT tau 0.8, magic 3/16, FP64 to 1e-8; then transport with a
+first-order surface reaction (Pe 10, Da 1) for 5000 steps.  Config 2: 128^3
+sphere pack (radii U(4, 8), porosity ~0.35, seed 0), D3Q19 in FP64, then
+Monod transport (Ks 0.5, Da 1, Pe 10).  Config 3: the same generator at
+256^3 and 384^3, populations stored in FP32 and in FP16 shifted by the
+weights, against the FP64 GPU solve.
+
+Bars: FP64 fields within 1e-12 relative (max |dv| / max |v|) of the
+oracle, the float32 cast within one ulp, the same stop iteration;
+permeability and outlet concentration within 1e-10 relative; float32
+transport fields bit for bit (given the same float32 velocity, which is
+what the drop-in hands the transport stage).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import prtlb_oracle as O
+from oracle import stokes_c as SC
+import prtlb
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def within_ulp(a, b):
+    ulp = np.spacing(np.abs(np.asarray(b, np.float32)))
+    return bool((np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)) <= ulp).all())
+
+
+def oracle_chem(ch):
+    """The oracle's own Chemistry with the attributes of a package one."""
+    return O.Chemistry(ch.names, ch.inlet, ch.c_init, ch.d_rel, ch.immobile, ks_donor=ch.ks_donor,
+                       ks_acceptor=ch.ks_acceptor, alpha=ch.alpha, yield_=ch.yield_,
+                       decay_lattice=ch.decay_lattice, biotic_on=ch.biotic_on,
+                       abiotic_index=ch.abiotic_index,
+                       abiotic_surface_only=ch.abiotic_surface_only,
+                       abiotic_product_index=ch.abiotic_product_index)
+
+
+def test_config1_pipeline_against_oracle():
+    """Config 1 through the drop-in entry points, every output checked."""
+    g = prtlb.disk_pack((128, 128), 6.0, 0.6, seed=0)
+    print("config 1: 128^2 disks, porosity %.3f" % (g == prtlb.PORE).mean(), flush=True)
+    info, oinfo = {}, {}
+    v32, v64 = prtlb.stokes_d2q9(g, tau_lb=0.8, magic=3.0 / 16.0, tol=1e-8, info=info,
+                                 return_f64=True)
+    v32o, (v64o, _) = O.stokes(g, tau_lb=0.8, magic=3.0 / 16.0, tol=1e-8, info=oinfo,
+                               return_f64=True)
+    assert info["flow_iters"] == oinfo["flow_iters"] and info["flow_converged"]
+    assert rel(v64, v64o) < 1e-12
+    assert within_ulp(v32, v32o)
+    k_o = O.permeability(v64o, g, 0.8, 2e-6)
+    assert abs(info["permeability"] / k_o - 1) < 1e-10, (info["permeability"], k_o)
+
+    ch = prtlb.chemistry.first_order(True)
+    kw = dict(pe=10.0, da=0.0, n_t=5, n_species=1, steps=5000, da_abio=1.0)
+    ainfo, oainfo = {}, {}
+    out, tn = prtlb.solve_adr(g, v32, chem=ch, info=ainfo, **kw)
+    outo, tno, Co = O.solve_adr(g, v32, chem=oracle_chem(ch), info=oainfo, return_state=True, **kw)
+    assert np.array_equal(out, outo) and np.array_equal(tn, tno)
+    assert ainfo["steps"] == oainfo["steps"] == 5000
+    oc, _ = O.outlet_concentration(Co, g)
+    assert oc[0] > 0
+    assert abs(float(ainfo["outlet_concentration"][0]) / oc[0] - 1) < 1e-10
+
+
+@pytest.fixture(scope="module")
+def pack128():
+    return prtlb.sphere_pack((128, 128, 128), (4.0, 8.0), 0.35, seed=0)
+
+
+def test_config2_stokes_fixed_steps_against_oracle(pack128):
+    """128^3 D3Q19 FP64, 100 iterations from rest: the field and the
+    permeability against the C restatement of the oracle."""
+    g = pack128
+    print("config 2: 128^3 spheres, porosity %.3f" % (g == prtlb.PORE).mean(), flush=True)
+    info = {}
+    v32, v64 = prtlb.stokes_d3q19(g, nit=100, force=2e-6, tol=0.0, info=info, return_f64=True)
+    v32o, (v64o, _) = SC.stokes(g, nit=100, force=2e-6, tol=0.0, return_f64=True)
+    assert rel(v64, v64o) < 1e-12
+    assert within_ulp(v32, v32o)
+    k_o = O.permeability(v64o, g, 0.8, 2e-6)
+    assert abs(info["permeability"] / k_o - 1) < 1e-10, (info["permeability"], k_o)
+
+
+def test_config2_converged_flow_and_transport_against_oracle(pack128):
+    """The flow run on the GPU under the config's stop rule (1e-8, the
+    default 30 000-iteration cap, which this pack reaches first: its mean
+    |u_x| still creeps by more than 1e-8 per 50 iterations); from that
+    developed state 60 more iterations on the GPU and in the oracle agree
+    to 1e-12 (permeability 1e-10).  Then 200 Monod steps on that flow, bit
+    for bit, with the outlet concentration to 1e-10; and the config's full
+    20 000 steps pass the reference's physics checks."""
+    g = pack128
+    S = prtlb.StokesSolver(g, force=2e-6, tau_lb=0.8, storage="fp64")
+    conv = S.solve(nit=30000, tol=1e-8)
+    print("config 2 flow: %d iterations, converged %s, change %.3g"
+          % (conv.iters, conv.converged, conv.change), flush=True)
+    assert conv.iters == 30000 or conv.converged
+    assert conv.change < 1e-6
+    # the stored post-collision populations, streamed: the oracle's f
+    fp = S.populations().cpu().numpy().reshape(19, -1)
+    e, _, _ = O.lattice(3)
+    src = O.pull_table(g.shape, e)
+    f_pre = fp[np.arange(19)[:, None], src].reshape((19,) + g.shape)
+    del fp, src
+    S.run(60)
+    v32, v64, sux = S.velocity(f64=True)
+    v64 = v64.cpu().numpy()
+    _, (v64o, _) = SC.stokes(g, nit=60, force=2e-6, tol=0.0, return_f64=True, f_init=f_pre)
+    assert rel(v64, v64o) < 1e-12
+    k_o = O.permeability(v64o, g, 0.8, 2e-6)
+    assert abs(S.permeability(sux) / k_o - 1) < 1e-10
+    vel = v32.cpu().numpy()
+    del S
+    torch.cuda.empty_cache()
+
+    ch = prtlb.chemistry.monod(0.5)
+    kw = dict(pe=10.0, da=1.0, n_t=3, n_species=1, steps=200)
+    ainfo = {}
+    out, tn = prtlb.solve_adr(g, vel, chem=ch, info=ainfo, **kw)
+    outo, tno, Co = O.solve_adr(g, vel, chem=oracle_chem(ch), return_state=True, **kw)
+    assert np.array_equal(out, outo) and np.array_equal(tn, tno)
+    oc, _ = O.outlet_concentration(Co, g)
+    assert abs(float(ainfo["outlet_concentration"][0]) - oc[0]) <= 1e-10 * max(abs(oc[0]), 1e-30)
+
+    full = {}
+    out, tn = prtlb.solve_adr(g, vel, chem=ch, info=full, **dict(kw, steps=20000))
+    assert full["steps"] == 20000
+    assert prtlb.check_physics_device(out, g, chem=ch, da=1.0) == []
+    assert float(out.min()) >= 0.0 and float(out.max()) <= 1.0 + 1e-5
+
+
+# Config 3: storage precision against the FP64 GPU solve after a fixed
+# iteration count.  Bounds: the errors measured on an MI355X (largest of the
+# three cases: FP32 velocity 1.6e-5, permeability 1.8e-6, concentration
+# 5.7e-6; FP16 2.5e-2, 5.5e-4, 1.8e-3) with headroom.  FP16 storage misses
+# the 1e-4 permeability bound at porosity 0.25; FP32 meets it with room.
+PREC_ITERS = 6000
+PREC_BOUNDS = {  # storage: (velocity, permeability, concentration)
+    "fp32": (1e-4, 2e-6, 1e-5),
+    "fp16": (5e-2, 1e-3, 3e-3),
+}
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("edge,phi", [(256, 0.25), (256, 0.45), (384, 0.25)])
+def test_config3_storage_precision(edge, phi):
+    g = prtlb.sphere_pack((edge, edge, edge), (4.0, 8.0), phi, seed=0)
+    print("config 3: %d^3 spheres, target porosity %.2f" % (edge, phi), flush=True)
+    ref = None
+    errs = {}
+    for st in ("fp64", "fp32", "fp16"):
+        S = prtlb.StokesSolver(g, force=2e-6, tau_lb=0.8, storage=st)
+        S.run(PREC_ITERS)
+        _, v64, sux = S.velocity(f64=True)
+        k = S.permeability(sux)
+        v = v64.cpu().numpy()
+        del S, v64
+        torch.cuda.empty_cache()
+        T = prtlb.TransportSolver(g, v if st == "fp64" else v.astype(np.float32), 10.0, 1.0, 1,
+                                  chem=prtlb.chemistry.monod(0.5),
+                                  precision="fp64" if st == "fp64" else "fp32")
+        T.step_n(2000)
+        c = T.current().double().cpu().numpy()
+        del T
+        torch.cuda.empty_cache()
+        if st == "fp64":
+            ref = (v, k, c)
+            continue
+        errs[st] = (rel(v, ref[0]), abs(k / ref[1] - 1), float(np.abs(c - ref[2]).max()))
+        print("  %s: velocity %.3g, permeability %.3g, concentration %.3g" % ((st,) + errs[st]),
+              flush=True)
+    for st, e in errs.items():
+        for got, bound in zip(e, PREC_BOUNDS[st]):
+            assert got <= bound, (st, e)
