@@ -119,7 +119,7 @@ int oracle_stokes_run(int q, int X, int Y, int Z, const uint8_t *solid, double *
     const int last = it == steps - 1;
     /* collide every node into tmp; |u_x| summed per x-plane in node order,
        then the planes in order, so the result does not depend on threads */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (N >= (1LL << 18))
     for (int x = 0; x < X; ++x) {
       double acc = 0.0;
       for (long long n = x * YZ; n < (x + 1) * YZ; ++n) {
@@ -157,7 +157,7 @@ int oracle_stokes_run(int q, int X, int Y, int Z, const uint8_t *solid, double *
       for (int x = 0; x < X; ++x) s_ux += plane[x];
     }
     /* pull-stream tmp into f, periodic */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (N >= (1LL << 18))
     for (int x = 0; x < X; ++x)
       for (int y = 0; y < Y; ++y)
         for (int z = 0; z < Z; ++z) {

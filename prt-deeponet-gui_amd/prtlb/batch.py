@@ -349,6 +349,8 @@ class AdrBatch:
     check_lag = 2
 
     def _launch(self, parity, steps):
+        if self.n_blocks < 1:          # every member's pore is empty: nothing to update
+            return
         s = stream_ptr(self.dev)
         for k in range(steps):
             call("prt_adr_batch_step", ptr(self.desc), ptr(self.blk_seg), self.n_blocks,
@@ -398,7 +400,7 @@ class AdrBatch:
             self._plan()
         todo = min(m.run.next_target() - m.run.n for m in self.members.values())
         left = todo
-        while self.use_graph and left >= self.GRAPH_STEPS:
+        while self.use_graph and self.n_blocks > 0 and left >= self.GRAPH_STEPS:
             self._graph_chunk()
             self.parity += self.GRAPH_STEPS
             left -= self.GRAPH_STEPS
@@ -410,7 +412,8 @@ class AdrBatch:
             n = m.run.n + todo
             m.T.steps_done = n
             if n == m.run.next_target():
-                m.run.arrive(n, defer=True)
+                # a member without pore voxels has no device sums to defer
+                m.run.arrive(n, defer=m.T.n_pore > 0)
             else:
                 m.run.n = n
             if not m.run.finished and (self.eager or len(m.run.pending) > self.check_lag):

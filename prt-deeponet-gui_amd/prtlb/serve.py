@@ -10,7 +10,9 @@ folded into the same launches as soon as memory frees up; a finished
 request leaves its batch and frees its memory at once.
 
 Each request's result is bit-identical to the same request served alone
-(tests/test_gpu_serve.py checks it): the batched kernels run the
+(tests/test_gpu_batch.py::test_server_result_same_alone_and_in_mix and
+::test_server_admission_under_a_binding_budget check it): the batched
+kernels run the
 single-geometry arithmetic, and every per-request reduction covers that
 request's nodes only.  The reference has no serving layer; its solver is a
 per-process function (3D/tools/prtlb_3d.py:348, :781).
@@ -33,6 +35,7 @@ class Request:
         self.key, self.g, self.pe, self.da = key, np.asarray(g), float(pe), float(da)
         self.kinetics, self.arrival, self.force, self.probe = kinetics, float(arrival), float(force), probe
         self.t_admit = self.t_done = None
+        self.round_admit = self.round_done = None   # scheduling rounds of admission and result
         self.result = None
 
 
@@ -53,12 +56,28 @@ class Server:
         self.n_t, self.adr_steps, self.max_steps = n_t, adr_steps, max_steps
         self.lb = {9: LbmBatch(9, storage, device=device), 19: LbmBatch(19, storage, device=device)}
         self.ab = AdrBatch(device=device)
-        # transport host set-ups (NumPy over the whole grid) run here, off
-        # the loop that keeps the device busy
-        self.pool = ThreadPoolExecutor(max_workers=4)
+        # transport host set-ups (NumPy over the whole grid) run on worker
+        # threads, off the loop that keeps the device busy; the pool lives
+        # for one run() (or until close())
+        self.pool = None
         self.waiting = deque()
         self.done = []
         self.reserved = {}
+        self.rounds = 0
+        self.peak_reserved = 0.0
+
+    def close(self):
+        """Stop the set-up workers; queued set-ups are cancelled."""
+        if self.pool is not None:
+            self.pool.shutdown(wait=False, cancel_futures=True)
+            self.pool = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
 
     def submit(self, req):
         self.waiting.append(req)
@@ -81,7 +100,9 @@ class Server:
                 break
             self.waiting.popleft()
             self.reserved[req.key] = need
+            self.peak_reserved = max(self.peak_reserved, self.in_use())
             req.t_admit = now
+            req.round_admit = self.rounds
             q = 9 if req.g.ndim == 2 else 19
             stage = "probe" if req.probe else "real"
             m = self.lb[q].add((req.key, stage), req.g, force=C.PROBE_FORCE if req.probe else req.force,
@@ -89,6 +110,7 @@ class Server:
             m.req = req
 
     def _round(self, now):
+        self.rounds += 1
         self._admit(now)
         for q, lb in self.lb.items():
             if not len(lb):
@@ -120,6 +142,7 @@ class Server:
                 req.result = dict(out=out, t_norm=tn, info=ainfo, permeability=req.permeability,
                                   velocity=req.velocity)
                 req.t_done = time.perf_counter()
+                req.round_done = self.rounds
                 self.reserved.pop(req.key, None)
                 self.done.append(req)
 
@@ -154,6 +177,17 @@ class Server:
         t_note = t0
         for r in sorted(requests, key=lambda r: r.arrival):
             self.submit(r)
+        if self.pool is None:
+            self.pool = ThreadPoolExecutor(max_workers=4)
+        try:
+            self._serve(t0, t_note, realtime, progress)
+        finally:
+            self.close()
+        for r in requests:
+            r.latency = (r.t_done - t0) - (r.arrival if realtime else 0.0)
+        return requests
+
+    def _serve(self, t0, t_note, realtime, progress):
         while not self.idle():
             if progress and time.perf_counter() - t_note > 20.0:
                 t_note = time.perf_counter()
@@ -167,9 +201,6 @@ class Server:
                 time.sleep(min(0.01, self.waiting[0].arrival - now))
                 continue
             self._round(now)
-        for r in requests:
-            r.latency = (r.t_done - t0) - (r.arrival if realtime else 0.0)
-        return requests
 
 
 def poisson_requests(n, rate=4.0, frac_2d=0.3, seed=0, size2d=(256, 1024), size3d=(64, 256),
@@ -197,6 +228,17 @@ def poisson_requests(n, rate=4.0, frac_2d=0.3, seed=0, size2d=(256, 1024), size3
 
 
 def latency_summary(reqs, wall):
+    """Throughput and latency percentiles over all requests, and separately
+    over those whose transport settled (the reference's stop rule) and
+    those that ran to the step cap."""
     lat = np.array([r.latency for r in reqs])
-    return dict(n=len(reqs), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
-                mean_s=float(lat.mean()), solves_per_hour=len(reqs) / wall * 3600.0, wall_s=wall)
+    conv = np.array([bool(r.result["info"].get("converged")) for r in reqs])
+
+    def pct(a, q):
+        return float(np.percentile(a, q)) if a.size else None
+
+    return dict(n=len(reqs), p50_s=pct(lat, 50), p99_s=pct(lat, 99), mean_s=float(lat.mean()),
+                solves_per_hour=len(reqs) / wall * 3600.0, wall_s=wall,
+                converged=int(conv.sum()), converged_fraction=float(conv.mean()),
+                p50_converged_s=pct(lat[conv], 50), p99_converged_s=pct(lat[conv], 99),
+                p50_capped_s=pct(lat[~conv], 50), p99_capped_s=pct(lat[~conv], 99))

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from oracle import prtlb_oracle as O
-from prtlb.adr import AdrRun
+from prtlb.adr import AdrRun, TransportSolver
 from prtlb import campaign as C
 import prtlb
 
@@ -48,6 +48,41 @@ class _NumpyDevice:
     def outlet_concentration(self, vel=None):
         return {"mean": np.zeros(self.ns)}
 
+    # ---- deferred checks: the sums "arrive" only after `lag` polls
+    lag = 1
+    steps_done = 0
+
+    @property
+    def n_pore(self):
+        return int(self.S.mf.sum())
+
+    _means_of = TransportSolver._means_of
+
+    def check_async(self, keep_state=True):
+        class _Host:
+            def __init__(s, a):
+                s.a = a
+
+            def numpy(s):
+                return s.a
+
+        class _Event:
+            def __init__(s, lag):
+                s.left = lag
+
+            def query(s):
+                s.left -= 1
+                return s.left < 0
+
+            def synchronize(s):
+                s.left = -1
+
+        sums = np.array([np.add.reduce(self.C[j][self.S.mf]) for j in range(self.ns)], np.float32)
+        return _Host(sums), _Event(self.lag), (self.C.copy() if keep_state else None)
+
+    def restore(self, state):
+        self.C = state.copy()
+
 
 @pytest.mark.parametrize("kw", [
     dict(pe=100.0, da=1.0, n_t=5, n_species=1),                   # auto stop
@@ -76,6 +111,48 @@ def test_adrrun_equals_reference_loop(kw):
     assert np.array_equal(out, out_o) and np.array_equal(tn, tn_o)
     for k in ("converged", "steps", "ladder", "repeated_snapshots", "dt"):
         assert info[k] == info_o[k], k
+
+
+@pytest.mark.parametrize("lag", [0, 1, 2, 3])
+@pytest.mark.parametrize("kw", [
+    dict(pe=100.0, da=1.0, n_t=5, n_species=1),                   # auto stop
+    dict(pe=40.0, da=2.0, n_t=6, n_species=4),                    # auto stop, 4 chemicals
+    dict(pe=30.0, da=1.0, n_t=4, n_species=4, steps=300),
+    dict(pe=20.0, da=0.5, n_t=3, n_species=2, max_steps=700)])    # cap
+def test_adrrun_deferred_checks_equal_reference_loop(kw, lag):
+    """Checks decided late (their sums arrive `lag` polls after the device
+    passed them, at most two left undecided, as solve_adr drives it): the
+    run that settled at an earlier check is put back to that step, so out,
+    t_norm and info equal the reference loop's."""
+    g = np.full((40, 6), O.PORE, np.uint8)
+    v = np.zeros((3, 40, 6), np.float32)
+    v[0] = 1.0
+    kw = dict(kw)
+    n_t, ns = kw.pop("n_t"), kw.pop("n_species")
+    steps, max_steps = kw.pop("steps", None), kw.pop("max_steps", 100000)
+    info_o = {}
+    out_o, tn_o = O.solve_adr(g, v, kw["pe"], kw["da"], n_t, n_species=ns, steps=steps,
+                              max_steps=max_steps, info=info_o)
+    T = _NumpyDevice(g, v, kw["pe"], kw["da"], ns, None, 0.0)
+    T.lag = lag
+    run = AdrRun(T, n_t, steps=steps, max_steps=max_steps)
+    late = 0
+    while not run.finished:
+        tgt = run.next_target()
+        T.step_n(tgt - run.n)
+        T.steps_done = tgt
+        run.arrive(tgt, defer=True)
+        if not run.finished:
+            late = max(late, len(run.pending))
+            run.resolve(block=len(run.pending) > 2)
+    info = {}
+    out, tn = run.result(info, kw["da"], 0.0)
+    assert np.array_equal(out, out_o) and np.array_equal(tn, tn_o)
+    for k in ("converged", "steps", "ladder", "repeated_snapshots", "dt"):
+        assert info[k] == info_o[k], k
+    assert T.steps_done == info_o["steps"]
+    if lag:
+        assert late >= 1
 
 
 def test_campaign_helpers():

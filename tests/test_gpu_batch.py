@@ -308,3 +308,62 @@ def test_dataset_stops_on_physics_complaint(monkeypatch, capsys):
     ds = D.build_dataset(n_geom=1, shape=(20, 16, 16), n_sets=2, n_times=3, radius=(2.0, 4.0),
                          adr_steps=100, seed=4, keep_complaints=True)
     assert list(ds["_complaints"]) == [1]
+
+
+def test_adr_batch_with_an_all_solid_member():
+    """A member without pore voxels next to a normal one, and alone: each
+    result equals its solve_adr."""
+    g_empty = np.full((12, 10), prtlb.SOLID, np.uint8)
+    v_empty = np.zeros((3, 12, 10), np.float32)
+    g = prtlb.disk_pack((40, 20), 3.0, 0.6, seed=6)
+    v = prtlb.stokes_d2q9(g, nit=1500, force=1e-5, tol=1e-6)
+    for members in (("empty", "pack"), ("empty",)):
+        ab = AdrBatch()
+        if "empty" in members:
+            ab.add("empty", g_empty, v_empty, 5.0, 1.0, 3, n_species=1, steps=300)
+        if "pack" in members:
+            ab.add("pack", g, v, 5.0, 1.0, 3, n_species=1, steps=300)
+        got = {}
+        while len(ab):
+            for m in ab.advance():
+                got[m.key] = ab.finish(m, {})
+        oe, te = prtlb.solve_adr(g_empty, v_empty, 5.0, 1.0, 3, n_species=1, steps=300)
+        assert np.array_equal(got["empty"][0], oe) and np.array_equal(got["empty"][1], te)
+        assert float(np.abs(oe).max()) == 0.0
+        if "pack" in members:
+            op, tp = prtlb.solve_adr(g, v, 5.0, 1.0, 3, n_species=1, steps=300)
+            assert np.array_equal(got["pack"][0], op) and np.array_equal(got["pack"][1], tp)
+
+
+def test_server_admission_under_a_binding_budget():
+    """An HBM budget smaller than the mix's footprint: requests wait, are
+    admitted as others leave (never holding more than the budget), and
+    each result equals the same request served alone."""
+    specs = [dict(shape=(40, 40, 40), radius=(3.0, 6.0), porosity=0.4, geom_seed=11),
+             dict(shape=(32, 32, 32), radius=(3.0, 6.0), porosity=0.45, geom_seed=12),
+             dict(shape=(96, 96), radius=6.0, porosity=0.6, geom_seed=13),
+             dict(shape=(36, 36, 36), radius=(3.0, 6.0), porosity=0.35, geom_seed=14)]
+    geoms = [C.make_geometry(s) for s in specs]
+    pes, das = [4.0, 12.0, 1.0, 30.0], [0.3, 2.0, 0.05, 1.0]
+
+    def reqs():
+        return [Request(i, geoms[i], pes[i], das[i], arrival=0.0) for i in range(len(geoms))]
+
+    probe = Server()
+    foot = [probe._footprint(r) for r in reqs()]
+    budget = 1.3 * max(foot)
+    assert sum(foot) > 2 * budget                     # the budget binds
+    with Server(hbm_budget=budget, nit=30000, adr_steps=None, max_steps=2000) as S:
+        mix = S.run(reqs(), realtime=False)
+        assert S.peak_reserved <= budget and S.in_use() == 0
+    by_key = {r.key: r for r in mix}
+    waited = [r for r in mix if any(o.round_done is not None and o.round_done <= r.round_admit
+                                    for o in mix if o is not r)]
+    assert waited, [(r.key, r.round_admit, r.round_done) for r in mix]
+    for k in range(len(geoms)):
+        alone = Server(nit=30000, adr_steps=None, max_steps=2000).run(
+            [Request(k, geoms[k], pes[k], das[k])], realtime=False)[0]
+        r = by_key[k]
+        assert np.array_equal(r.result["velocity"], alone.result["velocity"])
+        assert np.array_equal(r.result["out"], alone.result["out"])
+        assert r.result["info"]["steps"] == alone.result["info"]["steps"]
