@@ -1,16 +1,40 @@
 """What the transport solve reads about the chemicals.
 
-`Chemistry` keeps the constructor, attributes, `decay_for` and `truncate` of
-the reference's settings_and_units.Chemistry (3D/tools/settings_and_units.py:
-1834-1931) so that a caller's chemistry object works unchanged; solve_adr only
-reads attributes, so the reference's own Chemistry objects work too.
+solve_adr reads a chemistry object by attribute only (3D/tools/prtlb_3d.py:
+831-1018): per-chemical columns (feed at the inlet, NaN for none; initial
+value; diffusivity relative to the solute's and inside biofilm; whether it
+moves) and run-wide constants (half-saturations, stoichiometry, yield,
+decay, the reaction switches and indices, the reference length).  So a
+caller's object -- this package's Chemistry or the reference's own
+settings_and_units.Chemistry (3D/tools/settings_and_units.py:1834) -- works
+unchanged.  This Chemistry takes the same constructor arguments with the
+same defaults and exposes the same attributes, plus the two methods the
+solve calls, decay_for and truncate.
 """
+import copy
+
 import numpy as np
+
+# per-chemical columns and the dtype each is held in
+_COLUMNS = (("inlet", np.float32), ("c_init", np.float32), ("d_rel", np.float32),
+            ("immobile", bool))
+
+
+def _column(name, values, dtype, n):
+    col = np.asarray(values, dtype)
+    if col.shape != (n,):
+        raise ValueError("Chemistry: %s must give one value per chemical (%d chemicals, got "
+                         "shape %s)" % (name, n, col.shape))
+    return col
+
+
+def _opt(kind, v):
+    return None if v is None else kind(v)
 
 
 class Chemistry:
-    """Dimensionless chemistry: feeds, initial values, relative diffusivities,
-    mobility, kinetic constants and the switches the transport step reads."""
+    """Dimensionless chemistry: one column entry per chemical, then the
+    kinetic constants and switches of the run."""
 
     def __init__(self, names, inlet, c_init, d_rel, immobile, ks_donor=0.0,
                  ks_acceptor=0.15, alpha=0.4, yield_=0.04, da_decay=None,
@@ -20,77 +44,63 @@ class Chemistry:
                  biotic_on=True, l_ref_voxels=None, abiotic_product_index=None):
         self.names = list(names)
         n = len(self.names)
-        self.inlet = np.asarray(inlet, np.float32)
-        self.c_init = np.asarray(c_init, np.float32)
-        self.d_rel = np.asarray(d_rel, np.float32)
-        self.immobile = np.asarray(immobile, bool)
-        for arr, nm in ((self.inlet, "inlet"), (self.c_init, "c_init"),
-                        (self.d_rel, "d_rel"), (self.immobile, "immobile")):
-            if arr.shape != (n,):
-                raise ValueError("chemistry: %s has %d entries but there are %d "
-                                 "chemicals" % (nm, arr.size, n))
-        self.ks_donor = float(ks_donor)
-        self.ks_acceptor = float(ks_acceptor)
-        self.alpha = float(alpha)
-        self.yield_ = float(yield_)
-        self.da_decay = None if da_decay is None else float(da_decay)
-        self.decay_lattice = None if decay_lattice is None else float(decay_lattice)
-        self.source = source
-        self.biomass_coupled = bool(biomass_coupled)
-        self.abiotic_surface_only = bool(abiotic_surface_only)
+        given = {"inlet": inlet, "c_init": c_init, "d_rel": d_rel, "immobile": immobile}
+        for key, dtype in _COLUMNS:
+            setattr(self, key, _column(key, given[key], dtype, n))
+        # inside biofilm the pore diffusivities apply unless told otherwise
         self.d_biofilm_rel = (self.d_rel.copy() if d_biofilm_rel is None
-                              else np.asarray(d_biofilm_rel, np.float32))
-        self.biofilm_code = None if biofilm_code is None else int(biofilm_code)
-        self.seed_in_biofilm = bool(seed_in_biofilm)
+                              else _column("d_biofilm_rel", d_biofilm_rel, np.float32, n))
+        # kinetic constants (floats) and switches
+        self.ks_donor, self.ks_acceptor = float(ks_donor), float(ks_acceptor)
+        self.alpha, self.yield_ = float(alpha), float(yield_)
+        self.da_decay, self.decay_lattice = _opt(float, da_decay), _opt(float, decay_lattice)
+        self.biotic_on, self.biomass_coupled = bool(biotic_on), bool(biomass_coupled)
         self.abiotic_index = int(abiotic_index)
-        self.biotic_on = bool(biotic_on)
-        self.l_ref_voxels = None if l_ref_voxels is None else float(l_ref_voxels)
-        self.abiotic_product_index = (None if abiotic_product_index is None
-                                      else int(abiotic_product_index))
+        self.abiotic_product_index = _opt(int, abiotic_product_index)
+        self.abiotic_surface_only = bool(abiotic_surface_only)
+        self.biofilm_code, self.seed_in_biofilm = _opt(int, biofilm_code), bool(seed_in_biofilm)
+        self.l_ref_voxels = _opt(float, l_ref_voxels)
+        self.source = source
 
-    def decay_for(self, nx):
-        """Biomass decay in lattice units for a reference length nx."""
+    def decay_for(self, l_ref):
+        """Biomass decay per lattice step for reference length l_ref voxels:
+        the lattice value when one is set, else the decay Damkohler number
+        over the length (3D/tools/settings_and_units.py:1903)."""
         if self.decay_lattice is not None:
             return float(self.decay_lattice)
-        return float((self.da_decay or 0.0) / max(nx, 1))
+        da = 0.0 if self.da_decay is None else self.da_decay
+        return float(da / max(l_ref, 1))
 
     def truncate(self, ns):
-        """The same chemistry restricted to its first ns chemicals."""
-        ns = max(1, min(int(ns), len(self.names)))
-        return Chemistry(self.names[:ns], self.inlet[:ns], self.c_init[:ns],
-                         self.d_rel[:ns], self.immobile[:ns], self.ks_donor,
-                         self.ks_acceptor, self.alpha, self.yield_,
-                         self.da_decay, self.decay_lattice, self.source,
-                         self.biomass_coupled, self.abiotic_surface_only,
-                         self.d_biofilm_rel[:ns], self.biofilm_code,
-                         self.seed_in_biofilm, self.abiotic_index,
-                         self.biotic_on, self.l_ref_voxels,
-                         self.abiotic_product_index)
+        """A copy holding only the first ns chemicals (at least one, at
+        most all); the run-wide constants are shared."""
+        keep = max(1, min(int(ns), len(self.names)))
+        out = copy.copy(self)
+        out.names = self.names[:keep]
+        for key in ("inlet", "c_init", "d_rel", "immobile", "d_biofilm_rel"):
+            setattr(out, key, getattr(self, key)[:keep].copy())
+        return out
 
-    def describe(self):
-        rows = ["  %-6s %9s %9s %8s %9s" % ("name", "feed", "at t=0", "D/D_ref", "moves")]
-        for i, nm in enumerate(self.names):
-            fed = self.inlet[i]
-            rows.append("  %-6s %9s %9.4g %8.3g %9s"
-                        % (nm, "no feed" if fed != fed else "%.4g" % fed,
-                           self.c_init[i], self.d_rel[i],
-                           "no" if self.immobile[i] else "yes"))
-        return "\n".join(rows)
+
+# The built-in chemicals, in order (3D/tools/settings_and_units.py:1934-1951):
+# name, feed at the inlet (NaN: none), initial value, relative diffusivity
+# (pore and biofilm), immobile.
+_BUILT_IN = (
+    ("Ac", 1.0, 0.0, 1.0, False),       # electron donor
+    ("A", 1.0, 0.0, 1.0, False),        # electron acceptor
+    ("P", 0.0, 0.0, 1.0, False),        # product, held at zero at the inlet
+    ("Bio", np.nan, 0.1, 0.0, True),    # attached biomass, not fed
+)
 
 
 def default_chemistry(n_species=2):
-    """Donor, acceptor, product, attached biomass with the built-in constants
-    (3D/tools/settings_and_units.py:1934-1951)."""
-    ns = max(1, min(int(n_species), 4))
-    return Chemistry(
-        names=["Ac", "A", "P", "Bio"][:ns],
-        inlet=np.array([1.0, 1.0, 0.0, np.nan], np.float32)[:ns],
-        c_init=np.array([0.0, 0.0, 0.0, 0.1], np.float32)[:ns],
-        d_rel=np.array([1.0, 1.0, 1.0, 0.0], np.float32)[:ns],
-        d_biofilm_rel=np.array([1.0, 1.0, 1.0, 0.0], np.float32)[:ns],
-        immobile=np.array([False, False, False, True])[:ns],
-        ks_donor=0.0, ks_acceptor=0.15, alpha=0.4, yield_=0.04,
-        decay_lattice=0.006, source="built in")
+    """The first n_species (1..4) built-in chemicals with the built-in
+    constants: Ks_A 0.15, alpha 0.4, yield 0.04, lattice decay 0.006."""
+    rows = _BUILT_IN[:max(1, min(int(n_species), len(_BUILT_IN)))]
+    names, feed, start, drel, fixed = (list(c) for c in zip(*rows))
+    return Chemistry(names, feed, start, drel, fixed, d_biofilm_rel=drel, ks_donor=0.0,
+                     ks_acceptor=0.15, alpha=0.4, yield_=0.04, decay_lattice=0.006,
+                     source="built in")
 
 
 def first_order(da_surface_only=True):

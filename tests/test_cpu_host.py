@@ -302,3 +302,32 @@ def test_write_dataset_refuses_complaints(tmp_path):
     D.write_dataset(str(tmp_path / "good.npz"), ds)
     z = np.load(str(tmp_path / "good.npz"), allow_pickle=False)
     assert z["geom/material"].shape == (1,) + shape
+
+
+def test_chemistry_matches_the_oracle():
+    """The package's Chemistry holds what the oracle's restatement of the
+    reference's holds: built-in chemicals, truncation, decay, refusals."""
+    attrs = ("names", "inlet", "c_init", "d_rel", "d_biofilm_rel", "immobile", "ks_donor",
+             "ks_acceptor", "alpha", "yield_", "da_decay", "decay_lattice", "biotic_on",
+             "biomass_coupled", "abiotic_index", "abiotic_product_index", "abiotic_surface_only",
+             "biofilm_code", "seed_in_biofilm", "l_ref_voxels", "source")
+
+    def same(a, b):
+        for k in attrs:
+            x, y = getattr(a, k), getattr(b, k)
+            if isinstance(x, np.ndarray):
+                assert x.dtype == y.dtype and np.array_equal(x, y, equal_nan=x.dtype.kind == "f"), k
+            else:
+                assert x == y, k
+
+    for ns in (1, 2, 3, 4, 7):
+        same(prtlb.default_chemistry(ns), O.default_chemistry(ns))
+        same(prtlb.default_chemistry(4).truncate(ns), O.default_chemistry(4).truncate(ns))
+    kw = dict(ks_donor=0.3, da_decay=0.5, biofilm_code=3, l_ref_voxels=12, abiotic_product_index=1)
+    a = prtlb.Chemistry(["x", "y"], [1, np.nan], [0, 0.2], [1, 0.5], [0, 1], **kw)
+    b = O.Chemistry(["x", "y"], [1, np.nan], [0, 0.2], [1, 0.5], [0, 1], **kw)
+    same(a, b)
+    assert a.decay_for(40) == b.decay_for(40) == 0.5 / 40
+    assert a.truncate(1).inlet.shape == (1,) and a.inlet.shape == (2,)
+    with pytest.raises(ValueError):
+        prtlb.Chemistry(["x", "y"], [1.0], [0, 0], [1, 1], [0, 0])
