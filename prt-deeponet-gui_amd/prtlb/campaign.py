@@ -118,6 +118,30 @@ def _adr_kw(spec, kinetics):
     return dict(chem=chemmod.first_order(True), n_species=1, da=0.0, da_abio=spec["da"])
 
 
+def physics_view(out, chem, kinetics):
+    """(snapshots, chemistry) to hand check_physics for a run of `kinetics`.
+
+    check_physics gives each index its role (3D/tools/prtlb_3d.py:1684-1717):
+    0 the donor, capped at its feed; 1 the acceptor, capped at its own feed;
+    2 the product, capped at its feed plus the donor's.  The sorption
+    kinetics (build-defined) keep the sorbed S at index 1, where that rule
+    would cap it at zero.  Their snapshots are checked with S in the product
+    slot and an unfed, empty acceptor between, so every rule applies with
+    the bound S really has: the mobile chemical it is made from."""
+    if not kinetics.startswith("sorption"):
+        return out, chem
+    out = np.asarray(out)
+    empty = np.zeros_like(out[:, :1])
+    view = np.concatenate([out[:, :1], empty, out[:, 1:2]], axis=1)
+    ch = chemmod.Chemistry([chem.names[0], "(none)", chem.names[1]],
+                           [chem.inlet[0], np.nan, chem.inlet[1]],
+                           [chem.c_init[0], 0.0, chem.c_init[1]], [chem.d_rel[0], 0.0, 0.0],
+                           [False, True, True], biotic_on=False, abiotic_index=0,
+                           abiotic_product_index=2,
+                           abiotic_surface_only=chem.abiotic_surface_only)
+    return view, ch
+
+
 def run_sequential(specs, geoms, storage="fp64", tau=0.8, tol=1e-6, nit=30000,
                    kinetics="sorption_irreversible", n_t=5, adr_steps=None, max_steps=100000,
                    adr_precision="fp32", progress=None):

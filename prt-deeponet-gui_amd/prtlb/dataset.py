@@ -136,7 +136,9 @@ def build_dataset(n_geom=4, shape=(48, 48, 48), n_sets=2, n_times=5, porosity=(0
         settled[k] = bool(info.get("converged", True))
         u_ax = C.axial_mean(vel[i], mats[i])
         dP[k] = C.pressure_for_peclet(force[i], u_ax, p, shape[0], (tau - 0.5) / 3.0 * d_ratio)
-        bad = check_physics_device(out, mats[i], chem=chem, da=kwk["da"], da_abio=kwk.get("da_abio"))
+        view, vchem = C.physics_view(out, chem, kinetics)
+        bad = check_physics_device(view, mats[i], chem=vchem, da=kwk["da"],
+                                   da_abio=kwk.get("da_abio"))
         for msg in bad:
             print("     PHYSICS run %d (Pe=%.2f Da_bio=%.2f Da_abio=%.2f): %s"
                   % (k, p, kwk["da"], kwk.get("da_abio", 0.0), msg), flush=True)

@@ -331,3 +331,28 @@ def test_chemistry_matches_the_oracle():
     assert a.truncate(1).inlet.shape == (1,) and a.inlet.shape == (2,)
     with pytest.raises(ValueError):
         prtlb.Chemistry(["x", "y"], [1.0], [0, 0], [1, 1], [0, 0])
+
+
+def test_sorption_physics_view():
+    """Irreversible sorption keeps the sorbed S at index 1, the acceptor's
+    slot; checked through campaign.physics_view S is held to the bound it
+    really has (the fed solute), so a sound run is quiet and an S above
+    the feed is still named."""
+    from prtlb import campaign as C
+    ch = prtlb.chemistry.sorption_irreversible()
+    mat = np.full((30, 6), prtlb.PORE, np.uint8)
+    x = np.linspace(0.0, 1.0, 30, dtype=np.float32)[:, None] * np.ones((1, 6), np.float32)
+    out = np.zeros((3, 2, 30, 6), np.float32)
+    for t, f in enumerate((0.2, 0.6, 1.0)):
+        front = np.clip((f - x) * 3.0, 0.0, 1.0)
+        out[t, 0] = front * np.exp(-x)
+        out[t, 1] = 0.6 * front * (1 - np.exp(-x)) + 0.05 * t
+    raw = prtlb.check_physics(out, mat, chem=ch, da_abio=1.0)
+    assert any("S reaches" in s for s in raw)          # the acceptor's cap of zero
+    view, vch = C.physics_view(out, ch, "sorption_irreversible")
+    assert view.shape == (3, 3, 30, 6) and list(vch.names) == ["C", "(none)", "S"]
+    assert prtlb.check_physics(view, mat, chem=vch, da_abio=1.0) == []
+    out[2, 1, 5, 3] = 1.5
+    view, vch = C.physics_view(out, ch, "sorption_irreversible")
+    bad = prtlb.check_physics(view, mat, chem=vch, da_abio=1.0)
+    assert len(bad) == 1 and "S reaches 1.500, above the 1.000" in bad[0]

@@ -351,7 +351,7 @@ def test_server_admission_under_a_binding_budget():
 
     probe = Server()
     foot = [probe._footprint(r) for r in reqs()]
-    budget = 1.3 * max(foot)
+    budget = 1.05 * max(foot)
     assert sum(foot) > 2 * budget                     # the budget binds
     with Server(hbm_budget=budget, nit=30000, adr_steps=None, max_steps=2000) as S:
         mix = S.run(reqs(), realtime=False)
