@@ -305,6 +305,7 @@ def test_dataset_stops_on_physics_complaint(monkeypatch, capsys):
         D.build_dataset(n_geom=1, shape=(20, 16, 16), n_sets=2, n_times=3, radius=(2.0, 4.0),
                         adr_steps=100, seed=4)
     assert "PHYSICS run 1" in capsys.readouterr().out
+    calls.clear()
     ds = D.build_dataset(n_geom=1, shape=(20, 16, 16), n_sets=2, n_times=3, radius=(2.0, 4.0),
                          adr_steps=100, seed=4, keep_complaints=True)
     assert list(ds["_complaints"]) == [1]
