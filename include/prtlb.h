@@ -250,37 +250,37 @@ int prt_adr_step(const prt_adr_params *p, int nx, int ny, int nz,
                  float *c_out, void *stream);
 
 /* Pore table of a pore list, built once per transport solve (it depends on
- * the flags and the velocity only).  pore_idx holds n_pore flat voxel indices
- * (C order keeps every later load coalesced); for entry k the call writes
- * word[k] (the reach codes and face bits of that voxel) and
- * its face velocities: face[4k .. 4k+3] = (u_lo, u_hi) of axes 0 and 1, and
- * in 3D face[4 n_pore + 2k .. +1] = (u_lo, u_hi) of axis 2 (face holds
- * 2 nd n_pore values).  An entry that is not a pore voxel, or not below
- * nx*ny*nz, gets word 0xFFFFFFFF and is skipped by the list steps.  The face
- * velocities are the reference's open-face averages (3D/tools/prtlb_3d.py:
- * 1100-1135), computed with the same operations as prt_adr_step. */
+ * the flags and the velocity only): one device buffer of
+ * prt_adr_table_bytes(nd, fp64, n_pore) bytes (16-byte aligned).  pore_idx
+ * holds n_pore flat voxel indices in C order, the order the list steps
+ * need.  For entry k the table records the voxel index, its word (the reach
+ * codes and face bits of that voxel) and its face velocities, the
+ * reference's open-face averages (3D/tools/prtlb_3d.py:1100-1135) computed
+ * with the same operations as prt_adr_step, in aligned per-entry records
+ * (the layout is the library's own).  An entry that is not a pore voxel, or
+ * not below nx*ny*nz, is marked and skipped by the list steps.  Returns -1
+ * (and an error message) for a bad nd or count. */
+int64_t prt_adr_table_bytes(int nd, int fp64, int64_t n_pore);
 int prt_adr_pore_table(int nd, int nx, int ny, int nz, const uint8_t *flags,
                        const float *u, const int32_t *pore_idx, int64_t n_pore,
-                       uint32_t *word, float *face, void *stream);
+                       void *pore_table, void *stream);
 int prt_adr_pore_table_f64(int nd, int nx, int ny, int nz,
                            const uint8_t *flags, const double *u,
                            const int32_t *pore_idx, int64_t n_pore,
-                           uint32_t *word, double *face, void *stream);
+                           void *pore_table, void *stream);
 
-/* `steps` float32 steps with one thread per pore-list entry, geometry from
- * the pore table of the same list and grid: no lane works on solid, and no
- * flag or velocity grid is read.  Same arithmetic, same bits as prt_adr_run
- * on the listed voxels; voxels outside the list are never written, so
- * write_solid must be 0 (PRT_E_ARG). */
+/* `steps` float32 steps with one lane per pore-list entry, geometry from the
+ * pore table of the same C-ordered list and grid: no lane works on solid,
+ * and no flag or velocity grid is read.  Same arithmetic, same bits as
+ * prt_adr_run on the listed voxels; voxels outside the list are never
+ * written, so write_solid must be 0 (PRT_E_ARG). */
 int prt_adr_run_list(const prt_adr_params *p, int nx, int ny, int nz,
-                     const int32_t *pore_idx, const uint32_t *word,
-                     const float *face, int64_t n_pore, float *c_a, float *c_b,
-                     int steps, void *stream);
+                     const void *pore_table, int64_t n_pore, float *c_a,
+                     float *c_b, int steps, void *stream);
 
 /* One FP64 step over the pore list (FP64 pore table). */
 int prt_adr_step_list_f64(const prt_adr_params *p, int nx, int ny, int nz,
-                          const int32_t *pore_idx, const uint32_t *word,
-                          const double *face, int64_t n_pore,
+                          const void *pore_table, int64_t n_pore,
                           const double *c_in, double *c_out, void *stream);
 
 /* Same step in FP64 (the build's own double-precision variant). */
@@ -321,10 +321,8 @@ typedef struct prt_lbm_segment {
 typedef struct prt_adr_segment {
   const prt_adr_params *params;   /* host pointer, read by the plan call */
   int32_t nx, ny, nz, phase;
-  const int32_t *pore_idx;        /* device pore list and its pore table */
-  const uint32_t *word;           /* (prt_adr_pore_table) */
-  const float *face;
-  int64_t n_pore;
+  const void *pore_table;         /* device pore table (prt_adr_pore_table) */
+  int64_t n_pore;                 /* entries of its C-ordered pore list */
   float *c_a, *c_b;               /* device concentration buffers */
 } prt_adr_segment;
 
@@ -533,11 +531,10 @@ int prt_lbm_solver_velocity(prt_lbm_solver *h, double *v64, float *v32);
 void *prt_lbm_solver_newest(prt_lbm_solver *h);     /* newest population buffer */
 int prt_lbm_solver_destroy(prt_lbm_solver *h);
 
-/* A transport solve over a pore list and its table (prt_adr_pore_table);
- * the parameter block is copied into the handle. */
+/* A transport solve over the pore table of a pore list
+ * (prt_adr_pore_table); the parameter block is copied into the handle. */
 int prt_adr_solver_create(const prt_adr_params *p, int nx, int ny, int nz,
-                          const int32_t *pore_idx, const uint32_t *word,
-                          const float *face, long long n_pore, float *c_a,
+                          const void *pore_table, long long n_pore, float *c_a,
                           float *c_b, void *stream, prt_adr_solver **out);
 int prt_adr_solver_step_n(prt_adr_solver *h, int n);
 float *prt_adr_solver_newest(prt_adr_solver *h);    /* newest (ns, n) field */

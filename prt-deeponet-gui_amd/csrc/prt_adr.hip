@@ -1,12 +1,15 @@
 // prt_adr.hip -- fused finite-volume advection-diffusion-reaction step.
 //
-// One thread per voxel.  Everything the reference builds once per solve as
-// full-grid face tables (open/far faces, face velocities, harmonic face
-// diffusivities, div u, surface weights; 3D/tools/prtlb_3d.py:1100-1158) is
-// recomputed in registers from one flag byte per voxel, the cell-centred
-// velocity and a per-species 2x2 diffusivity table, so a step streams only
-//   C in (4 B/species) + C out (4 B/species) + u (4 B/axis) + flags (1 B)
-// per pore voxel; the radius-2 neighbours come from L1/L2.
+// Two launches of the same per-voxel update (prt_adr_core.h):
+//  - dense: one thread per box voxel; everything the reference builds once
+//    per solve as full-grid face tables (open/far faces, face velocities,
+//    harmonic face diffusivities, div u, surface weights;
+//    3D/tools/prtlb_3d.py:1100-1158) is recomputed in registers from one
+//    flag byte per voxel, the cell-centred velocity and a per-species 2x2
+//    diffusivity table;
+//  - pore list (the default): one lane per pore voxel, its geometry from a
+//    pore table built once per solve (two aligned records per voxel), the
+//    neighbours along z from neighbouring lanes, the others from L1/L2.
 //
 // The float32 instantiation reproduces the reference's float32 NumPy
 // arithmetic bit for bit: the same operations in the same order, no
@@ -32,46 +35,42 @@ __global__ __launch_bounds__(256) void adr_step_kernel(AdrArgs<R> a) {
   adr_voxel<R, ND, NSM>(a, (uint32_t)((x * a.Y + y) * a.Z + z), x, y, z, cnew);
 }
 
-// One thread per entry of a pore list (no lane of a wave is spent on solid),
-// geometry from the pore table.  Same update code, so the same bits.
+// The pore-list launch: lane l of global wave v holds list entry
+// v * kWaveOwn + l - kHalo (adr_lane), so no lane is spent on solid and the
+// last axis's neighbours come from neighbouring lanes.  Same update code as
+// the dense launch, so the same bits.
 template <typename R, int ND, int NSM, bool NTT>
-__global__ __launch_bounds__(256) void adr_list_kernel(AdrArgs<R> a, const int32_t *__restrict__ pidx,
-                                                      const uint32_t *__restrict__ word,
-                                                      const R *__restrict__ face, uint32_t npore) {
-  const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (k >= npore) return;
+__global__ __launch_bounds__(256) void adr_list_kernel(AdrArgs<R> a, const void *__restrict__ tab, uint32_t ta,
+                                                      uint32_t npore) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long long k = (long long)wave * kWaveOwn + (long long)lane - kHalo;
+  const bool own = lane >= (uint32_t)kHalo && lane < (uint32_t)(kHalo + kWaveOwn);
   R cnew[NSM];
-  adr_entry<R, ND, NSM, NTT>(a, pidx, word, face, k, npore, cnew);
+  adr_lane<R, ND, NSM, NTT>(a, tab, ta, k, npore, own, cnew);
 }
 
-
-// Pore table of a list: the word and face velocities of every entry, or
-// kPoreWordNone (and zero faces) for an entry that is not a pore voxel.
+// Pore table of a list: the record of every entry, or kPoreWordNone (and
+// zero faces) for an entry that is not a pore voxel.
 template <typename R, int ND>
 __global__ __launch_bounds__(256) void adr_table_kernel(AdrArgs<R> a, const int32_t *__restrict__ pidx,
-                                                       uint32_t npore, uint32_t *__restrict__ word,
-                                                       R *__restrict__ face) {
+                                                       uint32_t npore, uint32_t ta, void *__restrict__ tab) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= npore) return;
   const int n = pidx[k];
-  uint32_t w = kPoreWordNone;
-  vec4<R> f01 = {R(0), R(0), R(0), R(0)};
-  vec2<R> f2 = {R(0), R(0)};
+  VoxGeom<R, ND> g;
+  g.w = kPoreWordNone;
+#pragma unroll
+  for (int ax = 0; ax < ND; ++ax) g.u_lo[ax] = g.u_hi[ax] = R(0);
   if (n >= 0 && (long long)n < a.N) {
     const uint8_t fl = a.flags[n];
     if (fl & 1) {
       int x, y, z;
       adr_coords(a, n, x, y, z);
-      VoxGeom<R, ND> g;
       geom_dense<R, ND>(a, (uint32_t)n, fl, x, y, z, g);
-      w = g.w;
-      f01 = vec4<R>{g.u_lo[0], g.u_hi[0], g.u_lo[1], g.u_hi[1]};
-      f2 = vec2<R>{g.u_lo[ND - 1], g.u_hi[ND - 1]};
     }
   }
-  word[k] = w;
-  reinterpret_cast<vec4<R> *>(face)[k] = f01;
-  if (ND == 3) reinterpret_cast<vec2<R> *>(face + (size_t)4 * npore)[k] = f2;
+  tab_store<R, ND>(tab, ta, k, (uint32_t)n, g.w, g);
 }
 
 // ------------------------------------------------------------------ host side
@@ -108,36 +107,33 @@ static int launch(const prt_adr_params *p, const AdrArgs<R> &a, dim3 g, dim3 b, 
 }
 
 template <typename R, int ND, bool NTT>
-static void launch_list_ndt(const AdrArgs<R> &a, const int32_t *pidx, const uint32_t *word, const R *face,
-                            uint32_t npore, hipStream_t st) {
-  const dim3 g((npore + 255) / 256), b(256);
-  if (a.ns == 1) adr_list_kernel<R, ND, 1, NTT><<<g, b, 0, st>>>(a, pidx, word, face, npore);
-  else if (a.ns == 2) adr_list_kernel<R, ND, 2, NTT><<<g, b, 0, st>>>(a, pidx, word, face, npore);
-  else if (a.ns <= 4) adr_list_kernel<R, ND, 4, NTT><<<g, b, 0, st>>>(a, pidx, word, face, npore);
-  else adr_list_kernel<R, ND, 8, NTT><<<g, b, 0, st>>>(a, pidx, word, face, npore);
+static void launch_list_ndt(const AdrArgs<R> &a, const void *tab, uint32_t npore, hipStream_t st) {
+  const dim3 g(list_blocks(npore)), b(256);
+  const uint32_t ta = (uint32_t)tab_entries(npore);
+  if (a.ns == 1) adr_list_kernel<R, ND, 1, NTT><<<g, b, 0, st>>>(a, tab, ta, npore);
+  else if (a.ns == 2) adr_list_kernel<R, ND, 2, NTT><<<g, b, 0, st>>>(a, tab, ta, npore);
+  else if (a.ns <= 4) adr_list_kernel<R, ND, 4, NTT><<<g, b, 0, st>>>(a, tab, ta, npore);
+  else adr_list_kernel<R, ND, 8, NTT><<<g, b, 0, st>>>(a, tab, ta, npore);
 }
 
 template <typename R, int ND>
-static void launch_list_nd(const AdrArgs<R> &a, const int32_t *pidx, const uint32_t *word, const R *face,
-                           uint32_t npore, hipStream_t st) {
-  if (npore >= kNtTableEntries) launch_list_ndt<R, ND, true>(a, pidx, word, face, npore, st);
-  else launch_list_ndt<R, ND, false>(a, pidx, word, face, npore, st);
+static void launch_list_nd(const AdrArgs<R> &a, const void *tab, uint32_t npore, hipStream_t st) {
+  if (npore >= kNtTableEntries) launch_list_ndt<R, ND, true>(a, tab, npore, st);
+  else launch_list_ndt<R, ND, false>(a, tab, npore, st);
 }
 
-// Arguments of a list launch: parameters, grid, pore list and table.
+// Arguments of a list launch: parameters, grid and table.
 template <typename R>
-static int list_args(const char *who, const prt_adr_params *p, int nx, int ny, int nz, const int32_t *pidx,
-                     const uint32_t *word, const R *face, int64_t n_pore, const R *cin, R *cout,
-                     AdrArgs<R> &a) {
+static int list_args(const char *who, const prt_adr_params *p, int nx, int ny, int nz, const void *tab,
+                     int64_t n_pore, const R *cin, R *cout, AdrArgs<R> &a) {
   dim3 g, b;
   const int rc = adr_args_from_params<R>(p, nx, ny, nz, a, g, b);
   if (rc) return rc;
-  if (!cin || !cout || cin == cout || n_pore < 0 || n_pore > a.N ||
-      (n_pore > 0 && (!pidx || !word || !face))) {
+  if (!cin || !cout || cin == cout || n_pore < 0 || n_pore > a.N || (n_pore > 0 && !tab)) {
     set_error("%s: null or aliased buffers, or a bad pore count", who);
     return PRT_E_ARG;
   }
-  if ((unsigned long long)n_pore * 4ull * sizeof(R) >= (1ull << 32)) {
+  if ((unsigned long long)tab_entries(n_pore) * 16ull >= (1ull << 32)) {
     set_error("%s: pore list too long for the 32-bit table offsets", who);
     return PRT_E_RANGE;
   }
@@ -152,24 +148,25 @@ static int list_args(const char *who, const prt_adr_params *p, int nx, int ny, i
 
 template <typename R>
 static int table(int nd, int nx, int ny, int nz, const uint8_t *flags, const R *u, const int32_t *pidx,
-                 int64_t n_pore, uint32_t *word, R *face, hipStream_t st) {
+                 int64_t n_pore, void *tab, hipStream_t st) {
   AdrArgs<R> a;
   const int rc = adr_grid<R>(nd, nx, ny, nz, a);
   if (rc) return rc;
-  if (n_pore < 0 || n_pore > a.N || (n_pore > 0 && (!flags || !u || !pidx || !word || !face))) {
+  if (n_pore < 0 || n_pore > a.N || (n_pore > 0 && (!flags || !u || !pidx || !tab))) {
     set_error("prt_adr_pore_table: null buffers or a bad pore count");
     return PRT_E_ARG;
   }
-  if ((unsigned long long)n_pore * 4ull * sizeof(R) >= (1ull << 32)) {
+  if ((unsigned long long)tab_entries(n_pore) * 16ull >= (1ull << 32)) {
     set_error("prt_adr_pore_table: pore list too long for the 32-bit table offsets");
     return PRT_E_RANGE;
   }
   if (n_pore == 0) return PRT_OK;
   a.flags = flags;
   a.u = u;
+  const uint32_t ta = (uint32_t)tab_entries(n_pore);
   const dim3 g((unsigned)((n_pore + 255) / 256)), b(256);
-  if (nd == 2) adr_table_kernel<R, 2><<<g, b, 0, st>>>(a, pidx, (uint32_t)n_pore, word, face);
-  else adr_table_kernel<R, 3><<<g, b, 0, st>>>(a, pidx, (uint32_t)n_pore, word, face);
+  if (nd == 2) adr_table_kernel<R, 2><<<g, b, 0, st>>>(a, pidx, (uint32_t)n_pore, ta, tab);
+  else adr_table_kernel<R, 3><<<g, b, 0, st>>>(a, pidx, (uint32_t)n_pore, ta, tab);
   return launch_status("prt_adr_pore_table launch");
 }
 
@@ -214,23 +211,28 @@ extern "C" int prt_adr_run(const prt_adr_params *p, int nx, int ny, int nz, cons
   return PRT_OK;
 }
 
+extern "C" int64_t prt_adr_table_bytes(int nd, int fp64, int64_t n_pore) {
+  if ((nd != 2 && nd != 3) || n_pore < 0) {
+    set_error("prt_adr_table_bytes: nd must be 2 or 3 and n_pore >= 0");
+    return -1;
+  }
+  return tab_bytes(nd, fp64 != 0, n_pore);
+}
+
 extern "C" int prt_adr_pore_table(int nd, int nx, int ny, int nz, const uint8_t *flags, const float *u,
-                                  const int32_t *pore_idx, int64_t n_pore, uint32_t *word, float *face,
-                                  void *stream) {
-  return table<float>(nd, nx, ny, nz, flags, u, pore_idx, n_pore, word, face, (hipStream_t)stream);
+                                  const int32_t *pore_idx, int64_t n_pore, void *table_out, void *stream) {
+  return table<float>(nd, nx, ny, nz, flags, u, pore_idx, n_pore, table_out, (hipStream_t)stream);
 }
 
 extern "C" int prt_adr_pore_table_f64(int nd, int nx, int ny, int nz, const uint8_t *flags, const double *u,
-                                      const int32_t *pore_idx, int64_t n_pore, uint32_t *word, double *face,
-                                      void *stream) {
-  return table<double>(nd, nx, ny, nz, flags, u, pore_idx, n_pore, word, face, (hipStream_t)stream);
+                                      const int32_t *pore_idx, int64_t n_pore, void *table_out, void *stream) {
+  return table<double>(nd, nx, ny, nz, flags, u, pore_idx, n_pore, table_out, (hipStream_t)stream);
 }
 
-extern "C" int prt_adr_run_list(const prt_adr_params *p, int nx, int ny, int nz, const int32_t *pore_idx,
-                                const uint32_t *word, const float *face, int64_t n_pore, float *c_a, float *c_b,
-                                int steps, void *stream) {
+extern "C" int prt_adr_run_list(const prt_adr_params *p, int nx, int ny, int nz, const void *pore_table,
+                                int64_t n_pore, float *c_a, float *c_b, int steps, void *stream) {
   AdrArgs<float> a;
-  int rc = list_args<float>("prt_adr_run_list", p, nx, ny, nz, pore_idx, word, face, n_pore, c_a, c_b, a);
+  int rc = list_args<float>("prt_adr_run_list", p, nx, ny, nz, pore_table, n_pore, c_a, c_b, a);
   if (rc) return rc;
   if (steps < 0) {
     set_error("prt_adr_run_list: negative step count");
@@ -240,23 +242,21 @@ extern "C" int prt_adr_run_list(const prt_adr_params *p, int nx, int ny, int nz,
   for (int s = 0; s < steps; ++s) {
     a.cin = (s & 1) ? c_b : c_a;
     a.cout = (s & 1) ? c_a : c_b;
-    if (p->nd == 2) launch_list_nd<float, 2>(a, pore_idx, word, face, (uint32_t)n_pore, (hipStream_t)stream);
-    else launch_list_nd<float, 3>(a, pore_idx, word, face, (uint32_t)n_pore, (hipStream_t)stream);
+    if (p->nd == 2) launch_list_nd<float, 2>(a, pore_table, (uint32_t)n_pore, (hipStream_t)stream);
+    else launch_list_nd<float, 3>(a, pore_table, (uint32_t)n_pore, (hipStream_t)stream);
     rc = launch_status("prt_adr_run_list launch");
     if (rc) return rc;
   }
   return PRT_OK;
 }
 
-extern "C" int prt_adr_step_list_f64(const prt_adr_params *p, int nx, int ny, int nz, const int32_t *pore_idx,
-                                     const uint32_t *word, const double *face, int64_t n_pore,
-                                     const double *c_in, double *c_out, void *stream) {
+extern "C" int prt_adr_step_list_f64(const prt_adr_params *p, int nx, int ny, int nz, const void *pore_table,
+                                     int64_t n_pore, const double *c_in, double *c_out, void *stream) {
   AdrArgs<double> a;
-  const int rc = list_args<double>("prt_adr_step_list_f64", p, nx, ny, nz, pore_idx, word, face, n_pore, c_in,
-                                   c_out, a);
+  const int rc = list_args<double>("prt_adr_step_list_f64", p, nx, ny, nz, pore_table, n_pore, c_in, c_out, a);
   if (rc) return rc;
   if (n_pore == 0) return PRT_OK;
-  if (p->nd == 2) launch_list_nd<double, 2>(a, pore_idx, word, face, (uint32_t)n_pore, (hipStream_t)stream);
-  else launch_list_nd<double, 3>(a, pore_idx, word, face, (uint32_t)n_pore, (hipStream_t)stream);
+  if (p->nd == 2) launch_list_nd<double, 2>(a, pore_table, (uint32_t)n_pore, (hipStream_t)stream);
+  else launch_list_nd<double, 3>(a, pore_table, (uint32_t)n_pore, (hipStream_t)stream);
   return launch_status("prt_adr_step_list_f64 launch");
 }

@@ -131,14 +131,16 @@ __device__ inline T ldb(const T *base, uint32_t off) {
 // second value on a side sits r strides away and the first min(r, 1).
 // With a.mad24 each offset is one 24-bit multiply-add of the reach and the
 // stride in bytes; otherwise the strides are picked with selects.
-template <typename R, int ND>
+// Offsets of the axes below NAX (all of them by default; the list launch
+// takes the last axis from neighbouring lanes instead).
+template <typename R, int ND, int NAX = ND>
 __device__ inline void geom_index(const AdrArgs<R> &a, uint32_t n, uint32_t rc, VoxGeom<R, ND> &g) {
   const uint32_t nb = n * (uint32_t)sizeof(R);
   // 3D axes x, y, z; a 2D grid is (X, 1, Z) with axes x, z
   auto sbytes = [&](int ax) { return axis_stride(a, (ND == 2 && ax == 1) ? 2 : ax) * (int)sizeof(R); };
   if (a.mad24) {
 #pragma unroll
-    for (int ax = 0; ax < ND; ++ax) {
+    for (int ax = 0; ax < NAX; ++ax) {
       const int sb = sbytes(ax);
       const int rl = (rc >> (4 * ax)) & 3u, rh = (rc >> (4 * ax + 2)) & 3u;
       g.noff[ax][0] = nb + (uint32_t)__mul24(rl, -sb);
@@ -148,7 +150,7 @@ __device__ inline void geom_index(const AdrArgs<R> &a, uint32_t n, uint32_t rc, 
     }
   } else {
 #pragma unroll
-    for (int ax = 0; ax < ND; ++ax) {
+    for (int ax = 0; ax < NAX; ++ax) {
       const int sb = sbytes(ax);
       const int rl = (rc >> (4 * ax)) & 3u, rh = (rc >> (4 * ax + 2)) & 3u;
       const int om1 = rl ? sb : 0, op1 = rh ? sb : 0;
@@ -214,13 +216,28 @@ __device__ inline void geom_dense(const AdrArgs<R> &a, uint32_t n, uint8_t fl, i
   g.noff[ND - 1][3] = nb + rh * sb;
 }
 
-// Face velocities of list entry k: axes 0 and 1 as one 4-vector
-// (u_lo0, u_hi0, u_lo1, u_hi1) at face[4k], axis 2 (3D) as a 2-vector at
-// face[4*n_pore + 2k].
 template <typename R>
 using vec4 = R __attribute__((ext_vector_type(4)));
 template <typename R>
 using vec2 = R __attribute__((ext_vector_type(2)));
+
+// Pore table of a list of n entries: one device buffer of sections, each
+// holding ta entries (n rounded up to 64), every entry of a section one
+// aligned 8- or 16-byte record, so a wave reads each section as one
+// contiguous run.
+//   float32: section 0  {pore index, word, u_lo0, u_hi0}        16 B
+//            section 1  {u_lo1, u_hi1, u_lo2, u_hi2} (3D)      16 B
+//                       {u_lo1, u_hi1}              (2D)        8 B
+//   FP64:    section 0  {pore index, word}                       8 B
+//            section a+1 {u_lo_a, u_hi_a}, a < nd                16 B
+// (u_lo_a, u_hi_a: the velocities on the low and high face of the voxel
+// along axis a, 3D/tools/prtlb_3d.py:1100-1135.)
+inline long long tab_entries(long long n) { return (n + 63) / 64 * 64; }
+inline long long tab_bytes(int nd, bool fp64, long long n) {
+  const long long ta = tab_entries(n);
+  if (!fp64) return ta * (16 + (nd == 3 ? 16 : 8));
+  return ta * (8 + 16LL * nd);
+}
 
 // Pore lists (one list, or all members of a batch together) at least this
 // long stream their table with nontemporal loads: the table (32 B per pore
@@ -228,61 +245,100 @@ using vec2 = R __attribute__((ext_vector_type(2)));
 // between steps next to the concentration arrays.
 constexpr uint32_t kNtTableEntries = 1u << 22;
 
-// Load of a pore-table entry: nontemporal (NTT) for lists too large to
-// stay in the last-level cache between steps, ordinary otherwise.
+// Load of a table record: nontemporal (NTT) for lists too large to stay in
+// the last-level cache between steps, ordinary otherwise.
 template <bool NTT, typename T>
 __device__ inline T ldtab(const T *base, uint32_t i) {
   if constexpr (NTT) return ldnt32(base, i);
   else return ld32(base, i);
 }
 
-// Face velocities of list entry k into g (the loads are issued here, their
-// values used later).
+// The head of entry k: its voxel index and word (and, in float32, the face
+// velocities of axis 0, which share the record).
 template <typename R, int ND, bool NTT>
-__device__ inline void geom_faces(const R *face, uint32_t k, uint32_t n_pore, VoxGeom<R, ND> &g) {
-  const vec4<R> f01 = ldtab<NTT>(reinterpret_cast<const vec4<R> *>(face), k);
-  g.u_lo[0] = f01.x;
-  g.u_hi[0] = f01.y;
-  g.u_lo[1] = f01.z;
-  g.u_hi[1] = f01.w;
-  if (ND == 3) {
-    const vec2<R> f2 = ldtab<NTT>(reinterpret_cast<const vec2<R> *>(face + (size_t)4 * n_pore), k);
-    g.u_lo[ND - 1] = f2.x;
-    g.u_hi[ND - 1] = f2.y;
+__device__ inline void tab_head(const void *tab, uint32_t k, uint32_t &n, uint32_t &w, VoxGeom<R, ND> &g) {
+  if constexpr (sizeof(R) == 4) {
+    using U4 = uint32_t __attribute__((ext_vector_type(4)));
+    const U4 h = ldtab<NTT>(static_cast<const U4 *>(tab), k);
+    // components to scalars first: a bit cast of a vector component reads
+    // the start of the vector
+    const uint32_t lo = h.z, hi = h.w;
+    n = h.x;
+    w = h.y;
+    g.u_lo[0] = __builtin_bit_cast(R, lo);
+    g.u_hi[0] = __builtin_bit_cast(R, hi);
+  } else {
+    using U2 = uint32_t __attribute__((ext_vector_type(2)));
+    const U2 h = ldtab<NTT>(static_cast<const U2 *>(tab), k);
+    n = h.x;
+    w = h.y;
   }
 }
 
-// The step at one pore voxel n (list entry k) with geometry g: writes the
-// new values to cout and returns them in cnew.  An immobile chemical is
-// only ever read and written at the voxel itself, so with a.imm_list its
-// row holds the pore voxels in list order (entry k at k, rows of mobile
-// chemicals stay box arrays for the stencil): its load and store are then
-// contiguous across a wave instead of scattered over the box.
+// The face velocities of entry k not in its head.
+template <typename R, int ND, bool NTT>
+__device__ inline void tab_faces(const void *tab, uint32_t ta, uint32_t k, VoxGeom<R, ND> &g) {
+  const char *t = static_cast<const char *>(tab);
+  if constexpr (sizeof(R) == 4) {
+    const char *s1 = t + (size_t)ta * 16;
+    if constexpr (ND == 3) {
+      const vec4<R> f = ldtab<NTT>(reinterpret_cast<const vec4<R> *>(s1), k);
+      g.u_lo[1] = f.x; g.u_hi[1] = f.y; g.u_lo[2] = f.z; g.u_hi[2] = f.w;
+    } else {
+      const vec2<R> f = ldtab<NTT>(reinterpret_cast<const vec2<R> *>(s1), k);
+      g.u_lo[1] = f.x; g.u_hi[1] = f.y;
+    }
+  } else {
+#pragma unroll
+    for (int ax = 0; ax < ND; ++ax) {
+      const vec2<R> f = ldtab<NTT>(reinterpret_cast<const vec2<R> *>(t + (size_t)ta * (8 + 16 * ax)), k);
+      g.u_lo[ax] = f.x;
+      g.u_hi[ax] = f.y;
+    }
+  }
+}
+
+// Store of entry k's record (the table build).
+template <typename R, int ND>
+__device__ inline void tab_store(void *tab, uint32_t ta, uint32_t k, uint32_t n, uint32_t w, const VoxGeom<R, ND> &g) {
+  char *t = static_cast<char *>(tab);
+  if constexpr (sizeof(R) == 4) {
+    using U4 = uint32_t __attribute__((ext_vector_type(4)));
+    const R lo = g.u_lo[0], hi = g.u_hi[0];
+    reinterpret_cast<U4 *>(t)[k] = U4{n, w, __builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi)};
+    char *s1 = t + (size_t)ta * 16;
+    if constexpr (ND == 3)
+      reinterpret_cast<vec4<R> *>(s1)[k] = vec4<R>{g.u_lo[1], g.u_hi[1], g.u_lo[2], g.u_hi[2]};
+    else
+      reinterpret_cast<vec2<R> *>(s1)[k] = vec2<R>{g.u_lo[1], g.u_hi[1]};
+  } else {
+    using U2 = uint32_t __attribute__((ext_vector_type(2)));
+    reinterpret_cast<U2 *>(t)[k] = U2{n, w};
+#pragma unroll
+    for (int ax = 0; ax < ND; ++ax)
+      reinterpret_cast<vec2<R> *>(t + (size_t)ta * (8 + 16 * ax))[k] = vec2<R>{g.u_lo[ax], g.u_hi[ax]};
+  }
+}
+
+// The step at one pore voxel n (list entry k) with geometry g, given the
+// values its stencil reads: c[j] at the voxel and cn[j][axis][m2, m1, p1,
+// p2] as the reach codes place them (mobile chemicals; an immobile one is
+// only read at the voxel).  Writes the new values to cout and returns them
+// in cnew.  An immobile chemical is only ever read and written at the voxel
+// itself, so with a.imm_list its row holds the pore voxels in list order
+// (entry k at k, rows of mobile chemicals stay box arrays for the stencil):
+// its load and store are then contiguous across a wave instead of scattered
+// over the box.
 template <typename R, int ND, int NSM>
-__device__ inline void adr_update(const AdrArgs<R> &a, uint32_t n, uint32_t k, const VoxGeom<R, ND> &g, R *cnew) {
+__device__ inline void adr_apply(const AdrArgs<R> &a, uint32_t n, uint32_t k, const VoxGeom<R, ND> &g,
+                                 R (&c)[NSM], R (&cn)[NSM][ND][4], R *cnew) {
 #pragma clang fp contract(off)
   const int ns = a.ns;
   const uint32_t w = g.w;
   const int bf_self = (w >> kWordBiofilm) & 1;
-  // every value the stencil may need is loaded up front and unconditionally
-  // (the reach codes keep every offset inside the box)
-  R c[NSM], tr[NSM], cn[NSM][ND][4];
+  R tr[NSM];
 #pragma unroll
-  for (int j = 0; j < NSM; ++j) {
-    tr[j] = R(0);
-    c[j] = R(0);
-    if (j >= ns) continue;
-    const R *cj = a.cin + (size_t)j * a.N;
-    if (!a.mobile[j]) {
-      c[j] = ld32(cj, a.imm_list ? k : n);
-      continue;
-    }
-    c[j] = ld32(cj, n);
-#pragma unroll
-    for (int ax = 0; ax < ND; ++ax)
-#pragma unroll
-      for (int k = 0; k < 4; ++k) cn[j][ax][k] = ldb(cj, g.noff[ax][k]);
-  }
+  for (int j = 0; j < NSM; ++j) tr[j] = R(0);
   // the inlet pin (3D/tools/prtlb_3d.py:1398-1400) applied as the values are
   // read: a voxel on x = 0 that is read at all is a pore voxel.  Only voxels
   // with x <= 2 can see x = 0; along x the values read sit at x - reach,
@@ -448,6 +504,29 @@ __device__ inline void adr_update(const AdrArgs<R> &a, uint32_t n, uint32_t k, c
   }
 }
 
+// Every value the stencil of voxel n (list entry k) may need, loaded
+// unconditionally through the offsets of g (the reach codes keep every
+// offset inside the box), axes below NAX only.
+template <typename R, int ND, int NSM, int NAX = ND>
+__device__ inline void adr_gather(const AdrArgs<R> &a, uint32_t n, uint32_t k, const VoxGeom<R, ND> &g,
+                                  R (&c)[NSM], R (&cn)[NSM][ND][4]) {
+#pragma unroll
+  for (int j = 0; j < NSM; ++j) {
+    c[j] = R(0);
+    if (j >= a.ns) continue;
+    const R *cj = a.cin + (size_t)j * a.N;
+    if (!a.mobile[j]) {
+      c[j] = ld32(cj, a.imm_list ? k : n);
+      continue;
+    }
+    c[j] = ld32(cj, n);
+#pragma unroll
+    for (int ax = 0; ax < NAX; ++ax)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) cn[j][ax][m] = ldb(cj, g.noff[ax][m]);
+  }
+}
+
 // One voxel (flat index n = (x*Y + y)*Z + z) from the grids.  Returns false
 // (and writes nothing unless write_solid) outside the pore.
 template <typename R, int ND, int NSM>
@@ -462,26 +541,125 @@ __device__ inline bool adr_voxel(const AdrArgs<R> &a, uint32_t n, int x, int y, 
   }
   VoxGeom<R, ND> g;
   geom_dense<R, ND>(a, n, fl, x, y, z, g);
-  adr_update<R, ND, NSM>(a, n, n, g, cnew);
+  R c[NSM], cn[NSM][ND][4];
+  adr_gather<R, ND, NSM>(a, n, n, g, c, cn);
+  adr_apply<R, ND, NSM>(a, n, n, g, c, cn, cnew);
   return true;
 }
 
-// List entry k from the pore table.  Returns false for an entry marked not
-// pore (kPoreWordNone).
-template <typename R, int ND, int NSM, bool NTT = false>
-__device__ inline bool adr_entry(const AdrArgs<R> &a, const int32_t *pidx, const uint32_t *word, const R *face,
-                                 uint32_t k, uint32_t n_pore, R *cnew) {
-  // the word, the voxel index and the face velocities are all at entry k
-  // (k < n_pore, in range whatever the word says), so the three loads go out
-  // together instead of one after the other
+// ---- the pore-list launch
+//
+// A wave covers kWaveOwn consecutive list entries and kHalo more on each
+// side: lane l holds entry base + l - kHalo and updates it when
+// kHalo <= l < kHalo + kWaveOwn.  In a C-ordered pore list the neighbours
+// of voxel n along the last axis that the scheme reads, n - 2 .. n + 2, are
+// the entries k - 2 .. k + 2 whenever they are read at all (the reach codes
+// only point at an open neighbour, and an open neighbour is a pore voxel,
+// the next one in C order), so each lane loads only its own value and
+// takes those four from its neighbour lanes.
+constexpr int kHalo = 2;
+constexpr int kWaveOwn = 64 - 2 * kHalo;
+
+// Blocks of 256 threads (four waves) covering npore list entries.
+inline unsigned list_blocks(uint32_t npore) {
+  const unsigned waves = (npore + kWaveOwn - 1) / kWaveOwn;
+  return (waves + 3) / 4;
+}
+
+// v of the lane D places below (D > 0) or above (D < 0) this one; lanes
+// with no such lane get 0 (they are halo lanes, which update nothing).
+template <int D>
+__device__ inline uint32_t lane_shift32(uint32_t v) {
+  // DPP wave_shr:1 (0x138) / wave_shl:1 (0x130), applied |D| times
+  constexpr int ctrl = D > 0 ? 0x138 : 0x130;
+  int r = (int)v;
+#pragma unroll
+  for (int i = 0; i < (D > 0 ? D : -D); ++i) r = __builtin_amdgcn_update_dpp(0, r, ctrl, 0xF, 0xF, false);
+  return (uint32_t)r;
+}
+template <int D>
+__device__ inline float lane_shift(float v) {
+  return __builtin_bit_cast(float, lane_shift32<D>(__builtin_bit_cast(uint32_t, v)));
+}
+template <int D>
+__device__ inline double lane_shift(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint64_t lo = lane_shift32<D>((uint32_t)b), hi = lane_shift32<D>((uint32_t)(b >> 32));
+  return __builtin_bit_cast(double, lo | hi << 32);
+}
+
+// One lane of the pore-list launch: entry k (any value; lanes outside
+// [0, npore) only pass zeros on), updated when own.  Every lane of the wave
+// must call this together (the lane exchange needs them all).  Returns true
+// when the lane updated a pore voxel (cnew then holds its new values).
+template <typename R, int ND, int NSM, bool NTT>
+__device__ inline bool adr_lane(const AdrArgs<R> &a, const void *tab, uint32_t ta, long long k, uint32_t npore,
+                                bool own, R *cnew) {
+#pragma clang fp contract(off)
+  const bool valid = k >= 0 && k < (long long)npore;
+  const uint32_t kk = valid ? (uint32_t)k : 0u;
   VoxGeom<R, ND> g;
-  const uint32_t w = ldtab<NTT>(word, k);
-  const uint32_t n = (uint32_t)ldtab<NTT>(pidx, k);
-  geom_faces<R, ND, NTT>(face, k, n_pore, g);
-  if (w == kPoreWordNone || n >= (uint32_t)a.N) return false;
-  g.w = w;
-  geom_index<R, ND>(a, n, w, g);
-  adr_update<R, ND, NSM>(a, n, k, g, cnew);
+  uint32_t n = 0, w = kPoreWordNone;
+  // the whole record at once: head for every lane, faces for updating lanes
+  if (valid) tab_head<R, ND, NTT>(tab, kk, n, w, g);
+  if (valid && own) tab_faces<R, ND, NTT>(tab, ta, kk, g);
+  const bool pore = valid && w != kPoreWordNone && n < (uint32_t)a.N;
+  const bool upd = own && pore;
+  // then every value from memory in one round: the lane's own value of each
+  // mobile chemical (all lanes), and for updating lanes the neighbours
+  // along the other axes and the immobile chemicals
+  R c[NSM], cn[NSM][ND][4];
+#pragma unroll
+  for (int j = 0; j < NSM; ++j) {
+    c[j] = R(0);
+    if (j >= a.ns || !a.mobile[j]) continue;
+    if (pore) c[j] = ld32(a.cin + (size_t)j * a.N, n);
+  }
+  if (upd) {
+    g.w = w;
+    geom_index<R, ND, ND - 1>(a, n, w, g);
+#pragma unroll
+    for (int j = 0; j < NSM; ++j) {
+      if (j >= a.ns) continue;
+      const R *cj = a.cin + (size_t)j * a.N;
+      if (!a.mobile[j]) {
+        c[j] = ld32(cj, a.imm_list ? kk : n);
+        continue;
+      }
+#pragma unroll
+      for (int ax = 0; ax < ND - 1; ++ax)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) cn[j][ax][m] = ldb(cj, g.noff[ax][m]);
+    }
+  }
+  // the inlet pin on the own value (a voxel on x = 0 holds the feed;
+  // 3D/tools/prtlb_3d.py:1398-1400), so that the neighbours receive the
+  // values the scheme reads; then the exchange along the last axis
+  const bool x_is0 = ((w >> kWordX) & 3u) == 0;
+  R zl2[NSM], zl1[NSM], zh1[NSM], zh2[NSM];
+#pragma unroll
+  for (int j = 0; j < NSM; ++j) {
+    if (j >= a.ns || !a.mobile[j]) continue;
+    if (pore && x_is0 && a.pinned[j]) c[j] = a.fed[j];
+    zl2[j] = lane_shift<2>(c[j]);
+    zl1[j] = lane_shift<1>(c[j]);
+    zh1[j] = lane_shift<-1>(c[j]);
+    zh2[j] = lane_shift<-2>(c[j]);
+  }
+  if (!upd) return false;
+  // the last axis: the reach r on each side picks the voxel itself (0),
+  // the next lane (1) or the next two (2), as the offsets would
+  const uint32_t rl = (w >> (4 * (ND - 1))) & 3u, rh = (w >> (4 * (ND - 1) + 2)) & 3u;
+#pragma unroll
+  for (int j = 0; j < NSM; ++j) {
+    if (j >= a.ns || !a.mobile[j]) continue;
+    const R m1 = rl ? zl1[j] : c[j], p1 = rh ? zh1[j] : c[j];
+    cn[j][ND - 1][1] = m1;
+    cn[j][ND - 1][0] = rl == 2 ? zl2[j] : m1;
+    cn[j][ND - 1][2] = p1;
+    cn[j][ND - 1][3] = rh == 2 ? zh2[j] : p1;
+  }
+  adr_apply<R, ND, NSM>(a, n, kk, g, c, cn, cnew);
   return true;
 }
 

@@ -37,9 +37,8 @@ struct SegLbm {
 struct SegAdr {
   AdrArgs<float> a;
   float *ca, *cb;
-  const int32_t *pidx;   // pore list and its pore table
-  const uint32_t *word;
-  const float *face;
+  const void *tab;       // pore table of the segment's list
+  uint32_t ta;           // its entries per section (tab_entries)
   int npts;              // list entries
   int phase, blk0, nblk, nd;
 };
@@ -128,6 +127,8 @@ __global__ __launch_bounds__(256) void lbm_batch_sparse_kernel(const SegLbm *__r
   }
 }
 
+// Transport members: block b of member s covers the list entries of its
+// four waves, kWaveOwn each (adr_lane), so every wave sits in one member.
 template <int NSM, bool RED, bool NTT>
 __global__ __launch_bounds__(256) void adr_batch_kernel(const SegAdr *__restrict__ segs,
                                                         const int *__restrict__ blk_seg, int parity,
@@ -142,12 +143,13 @@ __global__ __launch_bounds__(256) void adr_batch_kernel(const SegAdr *__restrict
   const bool newest_a = ((parity ^ d.phase) & 1) == 0;
   a.cin = newest_a ? d.ca : d.cb;
   a.cout = newest_a ? d.cb : d.ca;
-  const uint32_t k = (blk - (uint32_t)d.blk0) * 256 + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = (blk - (uint32_t)d.blk0) * 4u + (threadIdx.x >> 6);
+  const long long k = (long long)wave * kWaveOwn + (long long)lane - kHalo;
+  const bool own = lane >= (uint32_t)kHalo && lane < (uint32_t)(kHalo + kWaveOwn);
   float cnew[NSM];
-  bool pore = false;
-  if (k < (uint32_t)d.npts)
-    pore = d.nd == 3 ? adr_entry<float, 3, NSM, NTT>(a, d.pidx, d.word, d.face, k, (uint32_t)d.npts, cnew)
-                     : adr_entry<float, 2, NSM, NTT>(a, d.pidx, d.word, d.face, k, (uint32_t)d.npts, cnew);
+  const bool pore = d.nd == 3 ? adr_lane<float, 3, NSM, NTT>(a, d.tab, d.ta, k, (uint32_t)d.npts, own, cnew)
+                              : adr_lane<float, 2, NSM, NTT>(a, d.tab, d.ta, k, (uint32_t)d.npts, own, cnew);
   if (RED) {
     // per-block pore sums of every chemical (FP64), NSM slots per block
 #pragma unroll
@@ -298,8 +300,7 @@ extern "C" int prt_adr_batch_plan(const prt_adr_segment *segs, int nseg, void *d
   int nsm = 1;
   for (int s = 0; s < nseg; ++s) {
     const prt_adr_segment &g = segs[s];
-    if (!g.params || !g.c_a || !g.c_b || g.c_a == g.c_b ||
-        (g.n_pore > 0 && (!g.pore_idx || !g.word || !g.face))) {
+    if (!g.params || !g.c_a || !g.c_b || g.c_a == g.c_b || (g.n_pore > 0 && !g.pore_table)) {
       set_error("prt_adr_batch_plan: segment %d has null or aliased buffers", s);
       return PRT_E_ARG;
     }
@@ -307,21 +308,20 @@ extern "C" int prt_adr_batch_plan(const prt_adr_segment *segs, int nseg, void *d
     dim3 gr, bl;
     const int rc = adr_args_from_params<float>(g.params, g.nx, g.ny, g.nz, d[s].a, gr, bl);
     if (rc) return rc;
-    if (g.n_pore < 0 || g.n_pore > d[s].a.N || (unsigned long long)g.n_pore * 16ull >= (1ull << 32) ||
+    if (g.n_pore < 0 || g.n_pore > d[s].a.N || (unsigned long long)tab_entries(g.n_pore) * 16ull >= (1ull << 32) ||
         g.params->write_solid) {
       set_error("prt_adr_batch_plan: segment %d: pore count out of range or write_solid set", s);
       return PRT_E_ARG;
     }
     d[s].ca = g.c_a;
     d[s].cb = g.c_b;
-    d[s].pidx = g.pore_idx;
-    d[s].word = g.word;
-    d[s].face = g.face;
+    d[s].tab = g.pore_table;
+    d[s].ta = (uint32_t)tab_entries(g.n_pore);
     d[s].npts = (int)g.n_pore;
     d[s].phase = g.phase & 1;
     d[s].nd = g.params->nd;
     d[s].blk0 = (int)total;
-    d[s].nblk = d[s].npts > 0 ? blocks_of(d[s].npts) : 0;
+    d[s].nblk = d[s].npts > 0 ? (int)list_blocks((uint32_t)d[s].npts) : 0;
     total += d[s].nblk;
     nsm = g.params->ns > nsm ? g.params->ns : nsm;
   }
@@ -349,7 +349,7 @@ extern "C" int prt_adr_batch_step(const void *desc_dev, const int32_t *blk_seg_d
   const dim3 grid((unsigned)n_blocks), block(256);
   const bool red = partials != nullptr;
   // the members' tables together decide the load kind, as one list's does
-  const bool ntt = n_blocks * 256 >= (int64_t)kNtTableEntries;
+  const bool ntt = n_blocks * 4 * kWaveOwn >= (int64_t)kNtTableEntries;
 #define PRT_AB3(NN, R)                                                                       \
   do {                                                                                       \
     if (ntt) adr_batch_kernel<NN, R, true><<<grid, block, 0, st>>>(d, blk_seg_dev, parity, partials); \

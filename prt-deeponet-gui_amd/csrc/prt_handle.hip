@@ -41,9 +41,7 @@ struct prt_lbm_solver {
 struct prt_adr_solver {
   prt_adr_params params;
   int nx, ny, nz;
-  const int32_t *pore_idx;
-  const uint32_t *word;
-  const float *face;
+  const void *table;
   long long n_pore;
   float *c[2];
   int latest;
@@ -141,10 +139,10 @@ extern "C" int prt_lbm_solver_destroy(prt_lbm_solver *h) {
   return PRT_OK;
 }
 
-extern "C" int prt_adr_solver_create(const prt_adr_params *p, int nx, int ny, int nz, const int32_t *pore_idx,
-                                     const uint32_t *word, const float *face, long long n_pore, float *c_a,
+extern "C" int prt_adr_solver_create(const prt_adr_params *p, int nx, int ny, int nz, const void *pore_table,
+                                     long long n_pore, float *c_a,
                                      float *c_b, void *stream, prt_adr_solver **out) {
-  if (!out || !p || !c_a || !c_b || c_a == c_b || n_pore < 0 || (n_pore > 0 && (!pore_idx || !word || !face))) {
+  if (!out || !p || !c_a || !c_b || c_a == c_b || n_pore < 0 || (n_pore > 0 && !pore_table)) {
     prt::set_error("prt_adr_solver_create: bad arguments");
     return PRT_E_ARG;
   }
@@ -155,11 +153,11 @@ extern "C" int prt_adr_solver_create(const prt_adr_params *p, int nx, int ny, in
   }
   h->params = *p;
   h->nx = nx; h->ny = ny; h->nz = nz;
-  h->pore_idx = pore_idx; h->word = word; h->face = face; h->n_pore = n_pore;
+  h->table = pore_table; h->n_pore = n_pore;
   h->c[0] = c_a; h->c[1] = c_b; h->latest = 0;
   h->stream = stream;
   // validate once through a zero-step run (argument checks only)
-  const int rc = prt_adr_run_list(&h->params, nx, ny, nz, pore_idx, word, face, n_pore, c_a, c_b, 0, stream);
+  const int rc = prt_adr_run_list(&h->params, nx, ny, nz, pore_table, n_pore, c_a, c_b, 0, stream);
   if (rc) {
     delete h;
     return rc;
@@ -173,7 +171,7 @@ extern "C" int prt_adr_solver_step_n(prt_adr_solver *h, int n) {
     prt::set_error("prt_adr_solver_step_n: bad arguments");
     return PRT_E_ARG;
   }
-  const int rc = prt_adr_run_list(&h->params, h->nx, h->ny, h->nz, h->pore_idx, h->word, h->face, h->n_pore,
+  const int rc = prt_adr_run_list(&h->params, h->nx, h->ny, h->nz, h->table, h->n_pore,
                                   h->c[h->latest], h->c[1 - h->latest], n, h->stream);
   if (!rc && (n & 1)) h->latest = 1 - h->latest;
   return rc;

@@ -70,8 +70,8 @@ class AdrSegment(ctypes.Structure):
     """Mirror of prt_adr_segment."""
     _fields_ = [("params", ctypes.POINTER(AdrParams)), ("nx", ctypes.c_int32),
                 ("ny", ctypes.c_int32), ("nz", ctypes.c_int32), ("phase", ctypes.c_int32),
-                ("pore_idx", ctypes.c_void_p), ("word", ctypes.c_void_p), ("face", ctypes.c_void_p),
-                ("n_pore", ctypes.c_int64), ("c_a", ctypes.c_void_p), ("c_b", ctypes.c_void_p)]
+                ("pore_table", ctypes.c_void_p), ("n_pore", ctypes.c_int64),
+                ("c_a", ctypes.c_void_p), ("c_b", ctypes.c_void_p)]
 
 
 _P = ctypes.c_void_p
@@ -106,7 +106,7 @@ _SIGS = {
     "prt_lbm_solver_velocity": (_I, [_P, _P, _P]),
     "prt_lbm_solver_newest": (_P, [_P]),
     "prt_lbm_solver_destroy": (_I, [_P]),
-    "prt_adr_solver_create": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, ctypes.c_longlong,
+    "prt_adr_solver_create": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, ctypes.c_longlong,
                                    _P, _P, _P, _P]),
     "prt_adr_solver_step_n": (_I, [_P, _I]),
     "prt_adr_solver_newest": (_P, [_P]),
@@ -135,12 +135,11 @@ _SIGS = {
     "prt_check_checker": (_I, [_P, _P, _I, _I64, _I, _P, _P]),
     "prt_check_moments": (_I, [_P, _I, _I, _P, _I64, _I, _P, _P, _P]),
     "prt_check_copy_residual": (_I, [_P, _I, _I, _I, ctypes.c_float, _P, _I64, _I, _P, _P]),
-    "prt_adr_pore_table": (_I, [_I, _I, _I, _I, _P, _P, _P, _I64, _P, _P, _P]),
-    "prt_adr_pore_table_f64": (_I, [_I, _I, _I, _I, _P, _P, _P, _I64, _P, _P, _P]),
-    "prt_adr_run_list": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _I64, _P, _P,
-                              _I, _P]),
-    "prt_adr_step_list_f64": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _I64, _P,
-                                   _P, _P]),
+    "prt_adr_table_bytes": (_I64, [_I, _I, _I64]),
+    "prt_adr_pore_table": (_I, [_I, _I, _I, _I, _P, _P, _P, _I64, _P, _P]),
+    "prt_adr_pore_table_f64": (_I, [_I, _I, _I, _I, _P, _P, _P, _I64, _P, _P]),
+    "prt_adr_run_list": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _I64, _P, _P, _I, _P]),
+    "prt_adr_step_list_f64": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _I64, _P, _P, _P]),
     "prt_lbm_batch_desc_size": (_I64, []),
     "prt_adr_batch_desc_size": (_I64, []),
     "prt_lbm_batch_plan": (_I, [_I, _I, _P, _I, _P, _P, _I64, ctypes.POINTER(_I64)]),

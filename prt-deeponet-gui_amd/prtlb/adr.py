@@ -16,7 +16,7 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import AdrParams, PrtError, call, pairwise_plan
+from ._lib import AdrParams, PrtError, call, lib, pairwise_plan
 from .chemistry import default_chemistry
 from .device import device as _device
 from .device import ptr, stream_ptr
@@ -256,13 +256,15 @@ class TransportSolver:
         self.latest = 0
         self.steps_done = 0
         self.pore_idx = torch.from_numpy(self.pore_idx_host).to(self.dev)
-        # pore table: per pore voxel its face bits / edge codes (uint32 kept
-        # in an int32 tensor) and face velocities, built once from flags and u
-        self.word = torch.empty(self.n_pore, dtype=torch.int32, device=self.dev)
-        self.face = torch.empty(2 * self.nd * self.n_pore, dtype=self.u.dtype, device=self.dev)
-        call("prt_adr_pore_table" if self.u.dtype == torch.float32 else "prt_adr_pore_table_f64",
+        # pore table: per pore voxel its index, face bits / reach codes and
+        # face velocities, built once from flags and u (an opaque buffer of
+        # aligned per-entry records)
+        fp64 = self.u.dtype == torch.float64
+        nbytes = int(lib().prt_adr_table_bytes(self.nd, int(fp64), self.n_pore))
+        self.table = torch.empty(max(nbytes, 16), dtype=torch.uint8, device=self.dev)
+        call("prt_adr_pore_table_f64" if fp64 else "prt_adr_pore_table",
              self.nd, *self.dims, ptr(self.flags), ptr(self.u), ptr(self.pore_idx), self.n_pore,
-             ptr(self.word), ptr(self.face), self._stream())
+             ptr(self.table), self._stream())
         self._plan = None
         self.means_scratch = None
         self.use_graph = self.n < (1 << 20)
@@ -346,8 +348,8 @@ class TransportSolver:
         """`steps` float32 steps a -> b -> ... (pore-list launch by default)."""
         pp = ctypes.byref(self.params)
         if self.pore_list:
-            call("prt_adr_run_list", pp, *self.dims, ptr(self.pore_idx), ptr(self.word),
-                 ptr(self.face), self.n_pore, ptr(a), ptr(b), int(steps), stream_ptr(self.dev))
+            call("prt_adr_run_list", pp, *self.dims, ptr(self.table), self.n_pore, ptr(a), ptr(b),
+                 int(steps), stream_ptr(self.dev))
         else:
             call("prt_adr_run", pp, *self.dims, ptr(self.flags), ptr(self.u), ptr(a), ptr(b),
                  int(steps), stream_ptr(self.dev))
@@ -382,8 +384,8 @@ class TransportSolver:
         else:
             for s in range(steps):
                 src, dst = (a, b) if s % 2 == 0 else (b, a)
-                call("prt_adr_step_list_f64", pp, *self.dims, ptr(self.pore_idx), ptr(self.word),
-                     ptr(self.face), self.n_pore, ptr(src), ptr(dst), self._stream())
+                call("prt_adr_step_list_f64", pp, *self.dims, ptr(self.table), self.n_pore,
+                     ptr(src), ptr(dst), self._stream())
         if steps % 2:
             self.latest = 1 - self.latest
         self.steps_done += steps

@@ -374,8 +374,8 @@ class AdrBatch:
         segs = (AdrSegment * len(mem))()
         for i, m in enumerate(mem):
             T = m.T
-            segs[i] = AdrSegment(ctypes.pointer(T.params), *T.dims, m.phase, ptr(T.pore_idx),
-                                 ptr(T.word), ptr(T.face), T.n_pore, ptr(T.c[0]), ptr(T.c[1]))
+            segs[i] = AdrSegment(ctypes.pointer(T.params), *T.dims, m.phase, ptr(T.table),
+                                 T.n_pore, ptr(T.c[0]), ptr(T.c[1]))
         dsz = lib().prt_adr_batch_desc_size()
         desc = np.zeros(dsz * len(mem), np.uint8)
         nb = ctypes.c_int64()

@@ -309,8 +309,8 @@ def test_solver_handles_match_stateless_calls():
     T.use_graph = False
     T.step_n(37)
     ha = ctypes.c_void_p()
-    _lib.call("prt_adr_solver_create", ctypes.byref(T.params), *T.dims, ptr(T.pore_idx), ptr(T.word),
-              ptr(T.face), T.n_pore, ptr(c_a), ptr(c_b), stream_ptr(T.dev), ctypes.byref(ha))
+    _lib.call("prt_adr_solver_create", ctypes.byref(T.params), *T.dims, ptr(T.table),
+              T.n_pore, ptr(c_a), ptr(c_b), stream_ptr(T.dev), ctypes.byref(ha))
     try:
         _lib.call("prt_adr_solver_step_n", ha, 20)
         _lib.call("prt_adr_solver_step_n", ha, 17)
