@@ -133,6 +133,22 @@ int prt_lbm_run(int q, int storage, int nx, int ny, int nz,
 /* Deterministic FP64 sum of n doubles (fixed order) into *out (device). */
 int prt_sum_f64(const double *x, int64_t n, double *out, void *stream);
 
+/* prt_lbm_run (steps >= 1) followed by prt_sum_f64 of the last iteration's
+ * per-block |u_x| partials into *sum_out (device): the FlowConvergence tick,
+ * with the same bits as those two calls (partials summed over
+ * prt_lbm_launch_blocks(q, nx, ny, nz, flags) entries), and no allocation.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (HIP graph capture, tight serving loops).  The partials live in the
+ * library's own device array of prt_lbm_static_partials_max() doubles (one
+ * per GPU, held until the process ends); a launch needing more returns
+ * PRT_E_RANGE.  Calls that overlap on one GPU share that array and race. */
+int64_t prt_lbm_static_partials_max(void);
+int prt_lbm_run_sum_static(int q, int storage, int nx, int ny, int nz,
+                           const uint8_t *solid, void *f_a, void *f_b,
+                           int steps, double omega_p, double omega_m,
+                           double force, double fp16_scale, int flags,
+                           double *sum_out, void *stream);
+
 /* Velocity from the post-collision state f_last (pull, density, Guo-shifted
  * momentum, zero in solid).  Replaces 3D/tools/prtlb_3d.py:451-455 and
  * 3D/tools/prtlb_2d.py:481-485.  With PRT_LBM_SKIP_SOLID in flags, f_prev is

@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 #include "prt_lbm_core.h"
 
@@ -469,6 +470,73 @@ extern "C" int prt_sum_f64(const double *x, int64_t n, double *out, void *stream
   }
   sum_f64_kernel<<<1, 1024, 0, (hipStream_t)stream>>>(x, n, out);
   return launch_status("prt_sum_f64 launch");
+}
+
+// ---- the allocation-free convergence tick (prt_lbm_run_sum_static)
+//
+// The per-block |u_x| partials of the tick's last iteration go to this
+// library's own device array, one copy per GPU held until the process
+// ends, instead of a caller buffer.  In host code the array's name stands
+// for a host-side symbol, not a device address: the device address of each
+// GPU's copy is looked up once per device (hipGetSymbolAddress, that device
+// current) and only that address is handed to kernels.
+constexpr long long kStaticPartials = 1LL << 20;   // doubles: 8 MiB per GPU
+static __device__ __align__(256) double scratch[kStaticPartials];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[kMaxDevices];
+
+// The current device's copy of scratch, or nullptr (error text set).
+static double *scratch_on_current_device() {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("prt_lbm_run_sum_static: no usable current device (%s)", hipGetErrorString(e));
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("prt_lbm_run_sum_static: hipGetSymbolAddress failed on device %d (%s)", dev,
+                hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_dev[dev] = static_cast<double *>(p);
+  }
+  return g_scratch_dev[dev];
+}
+
+extern "C" int64_t prt_lbm_static_partials_max(void) { return kStaticPartials; }
+
+extern "C" int prt_lbm_run_sum_static(int q, int storage, int nx, int ny, int nz, const uint8_t *solid,
+                                      void *f_a, void *f_b, int steps, double omega_p, double omega_m,
+                                      double force, double fp16_scale, int flags, double *sum_out,
+                                      void *stream) {
+  Geom g;
+  int rc = make_geom(q, nx, ny, nz, g, flags);
+  if (rc) return rc;
+  if (steps < 1 || !sum_out) {
+    set_error("prt_lbm_run_sum_static: needs at least one step and a device output");
+    return PRT_E_ARG;
+  }
+  const long long nblk = (long long)g.grid.x * g.grid.y * g.grid.z;
+  if (nblk > kStaticPartials) {
+    set_error("prt_lbm_run_sum_static: the launch has %lld blocks, more than the %lld partials of the "
+              "static scratch; use prt_lbm_run with a partials buffer",
+              nblk, kStaticPartials);
+    return PRT_E_RANGE;
+  }
+  double *part = scratch_on_current_device();
+  if (!part) return PRT_E_HIP;
+  // every block of the last iteration writes its partial: nothing to clear
+  rc = prt_lbm_run(q, storage, nx, ny, nz, solid, f_a, f_b, steps, omega_p, omega_m, force, fp16_scale, flags,
+                   part, stream);
+  if (rc) return rc;
+  sum_f64_kernel<<<1, 1024, 0, (hipStream_t)stream>>>(part, nblk, sum_out);
+  return launch_status("prt_lbm_run_sum_static sum launch");
 }
 
 extern "C" int prt_lbm_velocity(int q, int storage, int nx, int ny, int nz, const uint8_t *solid,

@@ -90,6 +90,8 @@ _SIGS = {
     "prt_lbm_partials_count": (_I64, [_I, _I, _I, _I]),
     "prt_lbm_run": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _I, _D, _D, _D, _D, _I, _P, _P]),
     "prt_sum_f64": (_I, [_P, _I64, _P, _P]),
+    "prt_lbm_static_partials_max": (_I64, []),
+    "prt_lbm_run_sum_static": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _I, _D, _D, _D, _D, _I, _P, _P]),
     "prt_lbm_velocity": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _I, _D, _D, _P, _P, _P, _P]),
     "prt_plane_flux": (_I, [_P, _I, _I, _I, _P, _P]),
     "prt_adr_step": (_I, [ctypes.POINTER(AdrParams), _I, _I, _I, _P, _P, _P, _P, _P]),
