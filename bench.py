@@ -3,15 +3,18 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
 A step is one fused D3Q19 TRT iteration (pull-stream + collide + Guo force +
-solid reversal) over the whole 128^3 sphere pack (radii U(4,8), porosity
-~0.35, seed 0), FP64 populations: the D3Q19 Stokes stage of the plain GPU
-drop-in workload.  Populations, mask and partial-sum buffers are resident in
-HBM before the timed region; K steps are timed with HIP events on the stream
-the kernels run on.  `value` is box-node MLUPS; the roofline object prices
-the 304 B/node algorithmic traffic against 8 TB/s.  Extra keys report the
-FP32 / FP16-shifted storages, the 256^3 grid, the fused transport step and
-the CPU baseline (this repo's NumPy oracle on the host, one core).
-Prints ONE JSON line.
+solid reversal) over the whole 256^3 sphere pack (radii U(4,8), porosity
+~0.35, seed 0), FP64 populations: the precision-variant workload's FP64
+case, where the 70% HBM target is stated.  Populations, mask and partial-sum
+buffers are resident in HBM before the timed region; K steps are timed with
+HIP events on the stream the kernels run on.  `value` is box-node MLUPS; the
+roofline object prices the 304 B/node algorithmic traffic against 8 TB/s.
+Extra keys report the FP32 / FP16-shifted storages, the 128^3 drop-in grid,
+the fluid lists, the fused transport step, the batched 64-rock campaign
+(solves/hour batched FP32 against one-at-a-time FP64, permeability error),
+the 48-request serving run (solves/hour, p50/p99 latency, converged share)
+and the CPU baseline (this repo's NumPy oracle on the host, one core, on a
+bounded sample of the same 256^3 step).  Prints ONE JSON line.
 """
 import argparse
 import json
@@ -173,24 +176,45 @@ def stream_copy(nbytes=2 * 1024 ** 3, reps=10):
     return out
 
 
-def cpu_baseline(budget_s=15.0):
+def host_cpu():
+    """CPU model, logical CPUs and the CPUs this process may run on."""
+    model = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return dict(model=model, logical_cpus=os.cpu_count(), usable_cpus=usable)
+
+
+def cpu_baseline(E=256, nit=2):
     """The NumPy oracle (a restatement of the reference's vectorised NumPy
-    loop) timed on the host: D3Q19 FP64 on a 64^3 sphere pack, one core."""
-    import numpy as np
+    loop, np.roll streaming) on a bounded sample of the headline workload:
+    the D3Q19 FP64 step on the same E^3 sphere pack, one core.  The set-up
+    and final moments are timed apart (a 1-iteration run) and taken out."""
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     import prtlb
     from oracle import prtlb_oracle as O
-    g = prtlb.sphere_pack((64, 64, 64), (4.0, 8.0), 0.35, seed=0)
+    g = prtlb.sphere_pack((E, E, E), (4.0, 8.0), 0.35, seed=0)
     t0 = time.perf_counter()
-    O.stokes(g, nit=2, force=2e-6, tol=0.0)
-    per = (time.perf_counter() - t0) / 2
-    nit = max(2, int(budget_s / max(per, 1e-3)))
-    t0 = time.perf_counter()
-    O.stokes(g, nit=nit, force=2e-6, tol=0.0)
-    dt = time.perf_counter() - t0
-    return dict(value=g.size * nit / dt / 1e6, unit="MLUPS", cores=1, kind="port",
-                sample="NumPy oracle D3Q19 FP64 TRT, 64^3 sphere pack, %d iterations "
-                       "(%.1f s) on %s" % (nit, dt, platform.processor() or platform.machine()))
+    O.stokes(g, nit=1, force=2e-6, tol=0.0)
+    t1 = time.perf_counter()
+    O.stokes(g, nit=1 + nit, force=2e-6, tol=0.0)
+    t2 = time.perf_counter()
+    per = max((t2 - t1) - (t1 - t0), 1e-9) / nit
+    cpu = host_cpu()
+    return dict(value=g.size / per / 1e6, unit="MLUPS", cores=1, kind="port",
+                sample="NumPy oracle D3Q19 FP64 TRT step, %d^3 sphere pack (the headline workload), "
+                       "%d iterations timed as the difference of a %d- and a 1-iteration run "
+                       "(%.1f s per iteration)" % (E, nit, 1 + nit, per),
+                cpu_model=cpu["model"], host_logical_cpus=cpu["logical_cpus"],
+                host_usable_cpus=cpu["usable_cpus"])
 
 
 def _oracle_stokes_worker(args):
@@ -229,14 +253,17 @@ def cpu_extras(procs=None):
     dt = time.perf_counter() - t0
     out["adr_2d_128sq_pore_voxel_steps_per_s_1core"] = round(int((g2 == 2).sum()) * 200 / dt, 1)
     g3 = prtlb.sphere_pack((64, 64, 64), (4.0, 8.0), 0.35, seed=0)
-    v3 = O.stokes(g3, nit=20, force=2e-6, tol=0.0)
+    t0 = time.perf_counter()
+    v3 = O.stokes(g3, nit=200, force=2e-6, tol=0.0)
+    dt = time.perf_counter() - t0
+    out["d3q19_64cube_200it_mlups_1core"] = round(g3.size * 200 / dt / 1e6, 3)
     t0 = time.perf_counter()
     O.solve_adr(g3, v3, 10.0, 1.0, 3, n_species=1, steps=200,
                 chem=O.Chemistry(names=["C"], inlet=[1.0], c_init=[0.0], d_rel=[1.0],
                                  immobile=[False], ks_donor=0.5, biotic_on=True))
     dt = time.perf_counter() - t0
     out["adr_3d_64cube_pore_voxel_steps_per_s_1core"] = round(int((g3 == 2).sum()) * 200 / dt, 1)
-    procs = procs or min(16, os.cpu_count() or 1)
+    procs = procs or min(16, host_cpu()["usable_cpus"] or 1)
     ctx = mp.get_context("spawn")            # fresh interpreters: no GPU state inherited
     t0 = time.perf_counter()
     with ctx.Pool(procs) as pool:
@@ -467,7 +494,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--grid", type=int, default=128, help="edge of the main cube")
+    ap.add_argument("--grid", type=int, default=256, help="edge of the main cube")
+    ap.add_argument("--no-studies", action="store_true",
+                    help="leave the 64-rock campaign and the 48-request serving run out")
     ap.add_argument("--storage", default="fp64", choices=["fp64", "fp32", "fp16"])
     ap.add_argument("--stream-only", action="store_true",
                     help="only the device-copy ceiling (counter calibration runs)")
@@ -545,16 +574,15 @@ def main():
         return
 
     E = args.grid
+    t_start = time.perf_counter()
     log("geometry %d^3 ..." % E)
     gmain = prtlb.sphere_pack((E, E, E), (4.0, 8.0), 0.35, seed=0)
     log("%s %d^3 ..." % (args.storage, E))
     main_r = time_lbm(gmain, args.storage, args.steps, args.warmup)
-    g128 = gmain if E == 128 else None
     extras = {}
     if not args.no_extras:
-        if g128 is None:
-            g128 = prtlb.sphere_pack((128, 128, 128), (4.0, 8.0), 0.35, seed=0)
-        for st in ("fp32", "fp16"):
+        g128 = prtlb.sphere_pack((128, 128, 128), (4.0, 8.0), 0.35, seed=0)
+        for st in ("fp64", "fp32", "fp16"):
             log(st, "128^3 ...")
             extras["d3q19_128_" + st] = time_lbm(g128, st, args.steps, args.warmup)
         log("fluid-only fp64 128^3 ...")
@@ -564,32 +592,39 @@ def main():
             log("fluid list", st, "128^3 ...")
             extras["d3q19_128_%s_fluid_list" % st] = time_fluid_list(g128, st, args.steps,
                                                                      args.warmup)
-        extras["d3q19_128_fp32_fluid_list_link32"] = time_fluid_list(g128, "fp32", args.steps,
-                                                                     args.warmup, link16=False)
-        log("geometry 256^3 ...")
-        g256 = prtlb.sphere_pack((256, 256, 256), (4.0, 8.0), 0.35, seed=0)
-        for st in ("fp64", "fp32", "fp16"):
-            log(st, "256^3 ...")
-            extras["d3q19_256_" + st] = time_lbm(g256, st, max(args.steps // 2, 20), args.warmup)
-        del g256
+        if E == 256:
+            for st in ("fp32", "fp16"):
+                log(st, "256^3 ...")
+                extras["d3q19_256_" + st] = time_lbm(gmain, st, args.steps, args.warmup)
+        del gmain
         log("D2Q9 2048^2 ...")
         g2d = prtlb.disk_pack((2048, 2048), 6.0, 0.6, seed=0, device="cuda")
         for st in ("fp64", "fp32"):
             extras["d2q9_2048sq_" + st] = time_lbm(g2d, st, args.steps, args.warmup)
         del g2d
-        log("transport 128^3 ...")
+        log("transport 128^3 / 256^3 ...")
         extras["adr_128_monod_f32"] = time_adr(g128, args.steps, args.warmup)
+        g256 = prtlb.sphere_pack((256, 256, 256), (4.0, 8.0), 0.35, seed=0)
+        extras["adr_256_monod_f32"] = time_adr(g256, max(args.steps // 2, 20), args.warmup)
+        del g256
+        torch.cuda.empty_cache()
         log("stream copy ...")
         extras["stream_copy"] = stream_copy()
         log("2D pipeline 128^2 ...")
         extras["d2q9_128sq_pipeline"] = config1_pipeline()
+        if not args.no_studies:
+            log("campaign 64 rocks ...")
+            extras["campaign_64"] = campaign_study(64, max_steps=args.adr_max_steps)
+            log("serve 48 requests ...")
+            extras["serve_48"] = serve_study(48, max_steps=args.adr_max_steps)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline()
+        cpu = cpu_baseline(E)
         if not args.no_extras:
             log("cpu extras ...")
             extras["cpu_extras"] = cpu_extras()
+    extras["bench_wall_s"] = round(time.perf_counter() - t_start, 1)
     line = {
         "metric": "D3Q19 TRT MLUPS & %HBM peak; batched solves/hr + p99 latency at fixed error",
         "value": round(main_r["mlups"], 1),
@@ -604,8 +639,9 @@ def main():
         "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f16 storage, f32 arithmetic"}[args.storage],
         "data": "synthetic: seeded %d^3 overlapping-sphere pack, radii U(4,8), porosity %.3f"
                 % (E, main_r["fluid_fraction"]),
-        "config": {"workload": "D3Q19 TRT Stokes step, %d^3 sphere pack, %s populations, "
-                               "dense box (every node updated)" % (E, args.storage.upper()),
+        "config": {"workload": "D3Q19 TRT Stokes step, %d^3 sphere pack (radii U(4,8), porosity "
+                               "0.35), %s populations, dense box (every node updated)"
+                               % (E, args.storage.upper()),
                    "grid": [E, E, E], "lattice": "D3Q19", "storage": args.storage,
                    "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "achieved": round(main_r["gbs"], 1),

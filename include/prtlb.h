@@ -286,7 +286,7 @@ int prt_adr_pore_table_f64(int nd, int nx, int ny, int nz,
                            void *pore_table, void *stream);
 
 /* `steps` float32 steps with one lane per pore-list entry, geometry from the
- * pore table of the same C-ordered list and grid: no lane works on solid,
+ * pore table of the same list and grid: no lane works on solid,
  * and no flag or velocity grid is read.  Same arithmetic, same bits as
  * prt_adr_run on the listed voxels; voxels outside the list are never
  * written, so write_solid must be 0 (PRT_E_ARG). */

@@ -161,8 +161,7 @@ def stokes(g, nit=30000, force=2e-6, tau_lb=0.8, magic=3.0 / 16.0, tol=1e-6,
     gf = float(force)
     E = e.astype(np.float64)
     f = np.repeat(w[:, None], g.size, 1)
-    src = pull_table(shape, e)
-    lanes = np.arange(q)[:, None]
+    axes = tuple(range(nd))
     guo = w * (1 - 0.5 * om_m) * 3.0 * gf * E[:, 0]
     conv = FlowConvergence(tol=tol, max_it=nit, every=every,
                            name="%dD flow" % nd)
@@ -185,7 +184,10 @@ def stokes(g, nit=30000, force=2e-6, tau_lb=0.8, magic=3.0 / 16.0, tol=1e-6,
               - om_m * (0.5 * (f - fo) - 0.5 * (feq - feo)))
         fp += guo[:, None]
         fp[:, solid] = fo[:, solid]
-        f = fp[lanes, src]
+        # periodic streaming, np.roll as 3D/tools/prtlb_3d.py:443
+        fp = fp.reshape((q,) + shape)
+        f = np.stack([np.roll(fp[i], tuple(int(c) for c in e[i]), axis=axes) for i in range(q)])
+        f = f.reshape(q, -1)
         if conv.done(it, np.abs(u[0][fluid]).mean() if fluid.any() else 0.0):
             break
     conv.report(info)

@@ -8,8 +8,8 @@
 //    flag byte per voxel, the cell-centred velocity and a per-species 2x2
 //    diffusivity table;
 //  - pore list (the default): one lane per pore voxel, its geometry from a
-//    pore table built once per solve (two aligned records per voxel), the
-//    neighbours along z from neighbouring lanes, the others from L1/L2.
+//    pore table built once per solve (aligned per-voxel records), the
+//    radius-2 neighbours from L1/L2.
 //
 // The float32 instantiation reproduces the reference's float32 NumPy
 // arithmetic bit for bit: the same operations in the same order, no
@@ -35,19 +35,16 @@ __global__ __launch_bounds__(256) void adr_step_kernel(AdrArgs<R> a) {
   adr_voxel<R, ND, NSM>(a, (uint32_t)((x * a.Y + y) * a.Z + z), x, y, z, cnew);
 }
 
-// The pore-list launch: lane l of global wave v holds list entry
-// v * kWaveOwn + l - kHalo (adr_lane), so no lane is spent on solid and the
-// last axis's neighbours come from neighbouring lanes.  Same update code as
-// the dense launch, so the same bits.
+// The pore-list launch: one thread per list entry, so no lane is spent on
+// solid; the geometry comes from the pore table.  Same update code as the
+// dense launch, so the same bits.
 template <typename R, int ND, int NSM, bool NTT>
 __global__ __launch_bounds__(256) void adr_list_kernel(AdrArgs<R> a, const void *__restrict__ tab, uint32_t ta,
                                                       uint32_t npore) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t wave = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const long long k = (long long)wave * kWaveOwn + (long long)lane - kHalo;
-  const bool own = lane >= (uint32_t)kHalo && lane < (uint32_t)(kHalo + kWaveOwn);
+  const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (k >= npore) return;
   R cnew[NSM];
-  adr_lane<R, ND, NSM, NTT>(a, tab, ta, k, npore, own, cnew);
+  adr_entry<R, ND, NSM, NTT>(a, tab, ta, k, cnew);
 }
 
 // Pore table of a list: the record of every entry, or kPoreWordNone (and
@@ -108,7 +105,7 @@ static int launch(const prt_adr_params *p, const AdrArgs<R> &a, dim3 g, dim3 b, 
 
 template <typename R, int ND, bool NTT>
 static void launch_list_ndt(const AdrArgs<R> &a, const void *tab, uint32_t npore, hipStream_t st) {
-  const dim3 g(list_blocks(npore)), b(256);
+  const dim3 g((npore + 255) / 256), b(256);
   const uint32_t ta = (uint32_t)tab_entries(npore);
   if (a.ns == 1) adr_list_kernel<R, ND, 1, NTT><<<g, b, 0, st>>>(a, tab, ta, npore);
   else if (a.ns == 2) adr_list_kernel<R, ND, 2, NTT><<<g, b, 0, st>>>(a, tab, ta, npore);

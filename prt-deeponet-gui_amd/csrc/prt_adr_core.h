@@ -131,16 +131,14 @@ __device__ inline T ldb(const T *base, uint32_t off) {
 // second value on a side sits r strides away and the first min(r, 1).
 // With a.mad24 each offset is one 24-bit multiply-add of the reach and the
 // stride in bytes; otherwise the strides are picked with selects.
-// Offsets of the axes below NAX (all of them by default; the list launch
-// takes the last axis from neighbouring lanes instead).
-template <typename R, int ND, int NAX = ND>
+template <typename R, int ND>
 __device__ inline void geom_index(const AdrArgs<R> &a, uint32_t n, uint32_t rc, VoxGeom<R, ND> &g) {
   const uint32_t nb = n * (uint32_t)sizeof(R);
   // 3D axes x, y, z; a 2D grid is (X, 1, Z) with axes x, z
   auto sbytes = [&](int ax) { return axis_stride(a, (ND == 2 && ax == 1) ? 2 : ax) * (int)sizeof(R); };
   if (a.mad24) {
 #pragma unroll
-    for (int ax = 0; ax < NAX; ++ax) {
+    for (int ax = 0; ax < ND; ++ax) {
       const int sb = sbytes(ax);
       const int rl = (rc >> (4 * ax)) & 3u, rh = (rc >> (4 * ax + 2)) & 3u;
       g.noff[ax][0] = nb + (uint32_t)__mul24(rl, -sb);
@@ -150,7 +148,7 @@ __device__ inline void geom_index(const AdrArgs<R> &a, uint32_t n, uint32_t rc, 
     }
   } else {
 #pragma unroll
-    for (int ax = 0; ax < NAX; ++ax) {
+    for (int ax = 0; ax < ND; ++ax) {
       const int sb = sbytes(ax);
       const int rl = (rc >> (4 * ax)) & 3u, rh = (rc >> (4 * ax + 2)) & 3u;
       const int om1 = rl ? sb : 0, op1 = rh ? sb : 0;
@@ -506,8 +504,8 @@ __device__ inline void adr_apply(const AdrArgs<R> &a, uint32_t n, uint32_t k, co
 
 // Every value the stencil of voxel n (list entry k) may need, loaded
 // unconditionally through the offsets of g (the reach codes keep every
-// offset inside the box), axes below NAX only.
-template <typename R, int ND, int NSM, int NAX = ND>
+// offset inside the box).
+template <typename R, int ND, int NSM>
 __device__ inline void adr_gather(const AdrArgs<R> &a, uint32_t n, uint32_t k, const VoxGeom<R, ND> &g,
                                   R (&c)[NSM], R (&cn)[NSM][ND][4]) {
 #pragma unroll
@@ -521,7 +519,7 @@ __device__ inline void adr_gather(const AdrArgs<R> &a, uint32_t n, uint32_t k, c
     }
     c[j] = ld32(cj, n);
 #pragma unroll
-    for (int ax = 0; ax < NAX; ++ax)
+    for (int ax = 0; ax < ND; ++ax)
 #pragma unroll
       for (int m = 0; m < 4; ++m) cn[j][ax][m] = ldb(cj, g.noff[ax][m]);
   }
@@ -547,119 +545,21 @@ __device__ inline bool adr_voxel(const AdrArgs<R> &a, uint32_t n, int x, int y, 
   return true;
 }
 
-// ---- the pore-list launch
-//
-// A wave covers kWaveOwn consecutive list entries and kHalo more on each
-// side: lane l holds entry base + l - kHalo and updates it when
-// kHalo <= l < kHalo + kWaveOwn.  In a C-ordered pore list the neighbours
-// of voxel n along the last axis that the scheme reads, n - 2 .. n + 2, are
-// the entries k - 2 .. k + 2 whenever they are read at all (the reach codes
-// only point at an open neighbour, and an open neighbour is a pore voxel,
-// the next one in C order), so each lane loads only its own value and
-// takes those four from its neighbour lanes.
-constexpr int kHalo = 2;
-constexpr int kWaveOwn = 64 - 2 * kHalo;
-
-// Blocks of 256 threads (four waves) covering npore list entries.
-inline unsigned list_blocks(uint32_t npore) {
-  const unsigned waves = (npore + kWaveOwn - 1) / kWaveOwn;
-  return (waves + 3) / 4;
-}
-
-// v of the lane D places below (D > 0) or above (D < 0) this one; lanes
-// with no such lane get 0 (they are halo lanes, which update nothing).
-template <int D>
-__device__ inline uint32_t lane_shift32(uint32_t v) {
-  // DPP wave_shr:1 (0x138) / wave_shl:1 (0x130), applied |D| times
-  constexpr int ctrl = D > 0 ? 0x138 : 0x130;
-  int r = (int)v;
-#pragma unroll
-  for (int i = 0; i < (D > 0 ? D : -D); ++i) r = __builtin_amdgcn_update_dpp(0, r, ctrl, 0xF, 0xF, false);
-  return (uint32_t)r;
-}
-template <int D>
-__device__ inline float lane_shift(float v) {
-  return __builtin_bit_cast(float, lane_shift32<D>(__builtin_bit_cast(uint32_t, v)));
-}
-template <int D>
-__device__ inline double lane_shift(double v) {
-  const uint64_t b = __builtin_bit_cast(uint64_t, v);
-  const uint64_t lo = lane_shift32<D>((uint32_t)b), hi = lane_shift32<D>((uint32_t)(b >> 32));
-  return __builtin_bit_cast(double, lo | hi << 32);
-}
-
-// One lane of the pore-list launch: entry k (any value; lanes outside
-// [0, npore) only pass zeros on), updated when own.  Every lane of the wave
-// must call this together (the lane exchange needs them all).  Returns true
-// when the lane updated a pore voxel (cnew then holds its new values).
+// List entry k (k below the list length) from the pore table: the whole
+// record is requested at once, then every value the stencil reads in one
+// round.  Returns false for an entry marked not pore (kPoreWordNone).
 template <typename R, int ND, int NSM, bool NTT>
-__device__ inline bool adr_lane(const AdrArgs<R> &a, const void *tab, uint32_t ta, long long k, uint32_t npore,
-                                bool own, R *cnew) {
-#pragma clang fp contract(off)
-  const bool valid = k >= 0 && k < (long long)npore;
-  const uint32_t kk = valid ? (uint32_t)k : 0u;
+__device__ inline bool adr_entry(const AdrArgs<R> &a, const void *tab, uint32_t ta, uint32_t k, R *cnew) {
   VoxGeom<R, ND> g;
-  uint32_t n = 0, w = kPoreWordNone;
-  // the whole record at once: head for every lane, faces for updating lanes
-  if (valid) tab_head<R, ND, NTT>(tab, kk, n, w, g);
-  if (valid && own) tab_faces<R, ND, NTT>(tab, ta, kk, g);
-  const bool pore = valid && w != kPoreWordNone && n < (uint32_t)a.N;
-  const bool upd = own && pore;
-  // then every value from memory in one round: the lane's own value of each
-  // mobile chemical (all lanes), and for updating lanes the neighbours
-  // along the other axes and the immobile chemicals
+  uint32_t n, w;
+  tab_head<R, ND, NTT>(tab, k, n, w, g);
+  tab_faces<R, ND, NTT>(tab, ta, k, g);
+  if (w == kPoreWordNone || n >= (uint32_t)a.N) return false;
+  g.w = w;
+  geom_index<R, ND>(a, n, w, g);
   R c[NSM], cn[NSM][ND][4];
-#pragma unroll
-  for (int j = 0; j < NSM; ++j) {
-    c[j] = R(0);
-    if (j >= a.ns || !a.mobile[j]) continue;
-    if (pore) c[j] = ld32(a.cin + (size_t)j * a.N, n);
-  }
-  if (upd) {
-    g.w = w;
-    geom_index<R, ND, ND - 1>(a, n, w, g);
-#pragma unroll
-    for (int j = 0; j < NSM; ++j) {
-      if (j >= a.ns) continue;
-      const R *cj = a.cin + (size_t)j * a.N;
-      if (!a.mobile[j]) {
-        c[j] = ld32(cj, a.imm_list ? kk : n);
-        continue;
-      }
-#pragma unroll
-      for (int ax = 0; ax < ND - 1; ++ax)
-#pragma unroll
-        for (int m = 0; m < 4; ++m) cn[j][ax][m] = ldb(cj, g.noff[ax][m]);
-    }
-  }
-  // the inlet pin on the own value (a voxel on x = 0 holds the feed;
-  // 3D/tools/prtlb_3d.py:1398-1400), so that the neighbours receive the
-  // values the scheme reads; then the exchange along the last axis
-  const bool x_is0 = ((w >> kWordX) & 3u) == 0;
-  R zl2[NSM], zl1[NSM], zh1[NSM], zh2[NSM];
-#pragma unroll
-  for (int j = 0; j < NSM; ++j) {
-    if (j >= a.ns || !a.mobile[j]) continue;
-    if (pore && x_is0 && a.pinned[j]) c[j] = a.fed[j];
-    zl2[j] = lane_shift<2>(c[j]);
-    zl1[j] = lane_shift<1>(c[j]);
-    zh1[j] = lane_shift<-1>(c[j]);
-    zh2[j] = lane_shift<-2>(c[j]);
-  }
-  if (!upd) return false;
-  // the last axis: the reach r on each side picks the voxel itself (0),
-  // the next lane (1) or the next two (2), as the offsets would
-  const uint32_t rl = (w >> (4 * (ND - 1))) & 3u, rh = (w >> (4 * (ND - 1) + 2)) & 3u;
-#pragma unroll
-  for (int j = 0; j < NSM; ++j) {
-    if (j >= a.ns || !a.mobile[j]) continue;
-    const R m1 = rl ? zl1[j] : c[j], p1 = rh ? zh1[j] : c[j];
-    cn[j][ND - 1][1] = m1;
-    cn[j][ND - 1][0] = rl == 2 ? zl2[j] : m1;
-    cn[j][ND - 1][2] = p1;
-    cn[j][ND - 1][3] = rh == 2 ? zh2[j] : p1;
-  }
-  adr_apply<R, ND, NSM>(a, n, kk, g, c, cn, cnew);
+  adr_gather<R, ND, NSM>(a, n, k, g, c, cn);
+  adr_apply<R, ND, NSM>(a, n, k, g, c, cn, cnew);
   return true;
 }
 

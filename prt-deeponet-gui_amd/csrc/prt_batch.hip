@@ -127,15 +127,17 @@ __global__ __launch_bounds__(256) void lbm_batch_sparse_kernel(const SegLbm *__r
   }
 }
 
-// Transport members: block b of member s covers the list entries of its
-// four waves, kWaveOwn each (adr_lane), so every wave sits in one member.
+// Transport members: block b of member s covers 256 consecutive entries of
+// its pore list.  With one or two chemicals the kernel is held to the
+// registers of eight waves per SIMD (left alone the compiler takes 68 vector
+// registers, seven waves; held, 60 and some scalars spilled to lanes).
 template <int NSM, bool RED, bool NTT>
-__global__ __launch_bounds__(256) void adr_batch_kernel(const SegAdr *__restrict__ segs,
-                                                        const int *__restrict__ blk_seg, int parity,
-                                                        double *partials) {
+__global__ __launch_bounds__(256, NSM <= 2 ? 8 : 1) void adr_batch_kernel(const SegAdr *__restrict__ segs,
+                                                                         const int *__restrict__ blk_seg,
+                                                                         int parity, double *partials) {
   __shared__ double lds[16];
   // round-robin block order here: measured faster than the XCD-contiguous
-  // order on the 64-rock campaign mix (580 against 604 us a step)
+  // order on the 64-rock campaign mix
   const uint32_t blk = blockIdx.x;
   const int s = __builtin_amdgcn_readfirstlane(blk_seg[blk]);
   const SegAdr &d = segs[s];
@@ -143,13 +145,12 @@ __global__ __launch_bounds__(256) void adr_batch_kernel(const SegAdr *__restrict
   const bool newest_a = ((parity ^ d.phase) & 1) == 0;
   a.cin = newest_a ? d.ca : d.cb;
   a.cout = newest_a ? d.cb : d.ca;
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t wave = (blk - (uint32_t)d.blk0) * 4u + (threadIdx.x >> 6);
-  const long long k = (long long)wave * kWaveOwn + (long long)lane - kHalo;
-  const bool own = lane >= (uint32_t)kHalo && lane < (uint32_t)(kHalo + kWaveOwn);
+  const uint32_t k = (blk - (uint32_t)d.blk0) * 256 + threadIdx.x;
   float cnew[NSM];
-  const bool pore = d.nd == 3 ? adr_lane<float, 3, NSM, NTT>(a, d.tab, d.ta, k, (uint32_t)d.npts, own, cnew)
-                              : adr_lane<float, 2, NSM, NTT>(a, d.tab, d.ta, k, (uint32_t)d.npts, own, cnew);
+  bool pore = false;
+  if (k < (uint32_t)d.npts)
+    pore = d.nd == 3 ? adr_entry<float, 3, NSM, NTT>(a, d.tab, d.ta, k, cnew)
+                     : adr_entry<float, 2, NSM, NTT>(a, d.tab, d.ta, k, cnew);
   if (RED) {
     // per-block pore sums of every chemical (FP64), NSM slots per block
 #pragma unroll
@@ -321,7 +322,7 @@ extern "C" int prt_adr_batch_plan(const prt_adr_segment *segs, int nseg, void *d
     d[s].phase = g.phase & 1;
     d[s].nd = g.params->nd;
     d[s].blk0 = (int)total;
-    d[s].nblk = d[s].npts > 0 ? (int)list_blocks((uint32_t)d[s].npts) : 0;
+    d[s].nblk = d[s].npts > 0 ? blocks_of(d[s].npts) : 0;
     total += d[s].nblk;
     nsm = g.params->ns > nsm ? g.params->ns : nsm;
   }
@@ -349,7 +350,7 @@ extern "C" int prt_adr_batch_step(const void *desc_dev, const int32_t *blk_seg_d
   const dim3 grid((unsigned)n_blocks), block(256);
   const bool red = partials != nullptr;
   // the members' tables together decide the load kind, as one list's does
-  const bool ntt = n_blocks * 4 * kWaveOwn >= (int64_t)kNtTableEntries;
+  const bool ntt = n_blocks * 256 >= (int64_t)kNtTableEntries;
 #define PRT_AB3(NN, R)                                                                       \
   do {                                                                                       \
     if (ntt) adr_batch_kernel<NN, R, true><<<grid, block, 0, st>>>(d, blk_seg_dev, parity, partials); \
