@@ -143,14 +143,21 @@ def time_adr(g, steps, warmup, kinetics="monod", pore_list=True, precision="fp32
                 frac=vps * bpv / PEAK_HBM, bytes_per_voxel=bpv, pore_voxels=T.n_pore)
 
 
-def measured_traffic(key):
+def measured_traffic(key, shape=None):
     """HBM bytes per launch of this workload from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (profiles/), or None if not measured."""
+    FETCH_SIZE / WRITE_SIZE summary (profiles/), for the launch shape the
+    tuner picked (entries measured per shape), or None if not measured."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
-        return json.load(open(path))[key]["traffic_bytes"]
+        entry = json.load(open(path))[key]
     except (OSError, ValueError, KeyError):
         return None
+    by_shape = entry.get("by_shape", {})
+    if shape is not None and str(shape) in by_shape:
+        return by_shape[str(shape)]
+    if shape is None or entry.get("launch_shape") == shape:
+        return entry.get("traffic_bytes")
+    return None
 
 
 def stream_copy(nbytes=2 * 1024 ** 3, reps=10):
@@ -516,6 +523,9 @@ def main():
     ap.add_argument("--fluid-list", action="store_true",
                     help="only the fluid-list step on the 128^3 pack (FP32 with 16- and 32-bit "
                          "links, FP64)")
+    ap.add_argument("--shapes", action="store_true",
+                    help="only the launch-shape tuner on the main grid: every candidate in order "
+                         "(2 + 5 iterations each), for counter runs that price each shape")
     ap.add_argument("--geometry", action="store_true",
                     help="only the geometry and check passes, GPU against host")
     ap.add_argument("--features", action="store_true",
@@ -547,6 +557,15 @@ def main():
         return
     if args.precision:
         print(json.dumps({"precision": precision_study()}), flush=True)
+        return
+    if args.shapes:
+        from prtlb import autotune
+        E = args.grid
+        g = prtlb.sphere_pack((E, E, E), (4.0, 8.0), 0.35, seed=0)
+        s = prtlb.StokesSolver(g, force=2e-6, storage=args.storage)
+        res = autotune.tune(s, steps=5, force=True)
+        print(json.dumps({"shapes": {"grid": E, "storage": args.storage, "best": res["tune"],
+                                     "order": res["order"], "ms": res["table"]}}), flush=True)
         return
     if args.features:
         print(json.dumps({"features": features_study()}), flush=True)
@@ -647,7 +666,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(main_r["gbs"], 1),
                      "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": round(main_r["frac"], 4),
-                     "traffic": measured_traffic("d3q19_%s_%d" % (args.storage, E))},
+                     "traffic": measured_traffic("d3q19_%s_%d" % (args.storage, E),
+                                                 main_r["launch_shape"])},
         "cpu_baseline": cpu,
         "mlups_fluid": round(main_r["mlups_fluid"], 1),
         "extras": extras,

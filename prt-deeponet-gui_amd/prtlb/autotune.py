@@ -75,16 +75,19 @@ def tune(s, steps=10, candidates=None, force=False):
         s.tune = _CACHE[key]["tune"]
         return _CACHE[key]
     table = {"default": time_shape(s, 0, steps)}
+    order = [("default", 0)]                 # candidates in the order they ran
     best, best_ms = 0, table["default"]
     for bz, by, npt, ntl, xcd in (candidates or CANDIDATES):
         if (1 << bz) * (1 << by) > 512 or ((ntl or xcd) and (s.flags or npt != 1)):
             continue
         t = lbm_tune(bz, by, npt, bool(ntl), bool(xcd))
         ms = time_shape(s, t, steps)
-        table["%dx%dx%d%s%s" % (1 << bz, 1 << by, npt, "nt" if ntl else "", "x" if xcd else "")] = ms
+        name = "%dx%dx%d%s%s" % (1 << bz, 1 << by, npt, "nt" if ntl else "", "x" if xcd else "")
+        table[name] = ms
+        order.append((name, t))
         if ms < best_ms:
             best, best_ms = t, ms
     s.tune = best
-    _CACHE[key] = {"tune": best, "ms": best_ms, "table": table}
+    _CACHE[key] = {"tune": best, "ms": best_ms, "table": table, "order": order}
     _save()
     return _CACHE[key]
