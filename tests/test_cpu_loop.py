@@ -171,3 +171,35 @@ def test_campaign_helpers():
     v[0] = 1e-6
     assert C.lowered_force(v, 2e-6) == 2e-6          # never raised
     assert C.pressure_for_peclet(1e-5, 0.01, 20.0, 64, 0.5) == pytest.approx(1e-5 * 20 / (0.01 * 64 / 0.5))
+
+
+def test_force_probe_and_peclet_scaling_by_linearity():
+    """The builders' probe rule and Peclet rescale (3D/tools/build_dataset_3d.py:
+    685-726, 3D/tools/settings_and_units.py:1349-1413), pinned by a slit, where
+    the flow is exactly linear in the body force (unidirectional flow has no
+    inertia): re-solving at campaign.lowered_force puts the peak speed at
+    Mach 0.02, and re-solving at pressure_for_peclet gives the wanted Peclet,
+    both to the solver's convergence.  Flows from the oracle's C iteration."""
+    from oracle import stokes_c as SC
+    h = 12
+    g = np.full((3, 2, h + 2), O.PORE, np.uint8)
+    g[:, :, 0] = g[:, :, -1] = O.SOLID
+
+    def flow(force):
+        return SC.stokes(g, nit=400000, force=force, tau_lb=0.8, tol=1e-13, return_f64=True)[1][0]
+
+    probe = flow(C.PROBE_FORCE)
+    f = C.lowered_force(probe, 1e-3)                  # a seed high enough to be lowered
+    assert f < 1e-3
+    v = flow(f)
+    mach = float(np.sqrt((v ** 2).sum(0)).max()) * np.sqrt(3.0)
+    assert abs(mach / 0.02 - 1) < 1e-8
+    assert C.lowered_force(probe, 1e-9) == 1e-9       # never raised
+    # Peclet: the axial pore mean of the seed flow, then the rescaled force
+    d_lat = 0.1 * 1e-3
+    u0 = C.axial_mean(v, g)
+    assert u0 == pytest.approx(float(v[0][g == O.PORE].mean()), rel=1e-15)
+    for pe in (0.3, 10.0, 250.0):
+        fp = C.pressure_for_peclet(f, u0, pe, g.shape[0], d_lat)
+        u1 = C.axial_mean(flow(fp), g)
+        assert abs(C.peclet_from_flow(u1, g.shape[0], d_lat) / pe - 1) < 1e-8
