@@ -214,6 +214,30 @@ __device__ inline void geom_dense(const AdrArgs<R> &a, uint32_t n, uint8_t fl, i
   g.noff[ND - 1][3] = nb + rh * sb;
 }
 
+// All ones when bit b of w is set, else zero (one sign-extending bit
+// extract).
+__device__ inline uint32_t bitmask(uint32_t w, int b) {
+  return (uint32_t)((int32_t)(w << (31 - b)) >> 31);
+}
+// m ? x : y for a mask m from bitmask: a bitwise select of the two values'
+// bits, so it is exact for every value (NaN included) and needs no compare.
+// In float32 it is written as the one bitfield-insert instruction: left to
+// itself the compiler turns the bit operations back into a bit test, a
+// compare and a conditional move per select.
+template <typename R>
+__device__ inline R bsel(uint32_t m, R x, R y) {
+  if constexpr (sizeof(R) == 4) {
+    const uint32_t xi = __builtin_bit_cast(uint32_t, x), yi = __builtin_bit_cast(uint32_t, y);
+    uint32_t r;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(xi), "v"(yi));
+    return __builtin_bit_cast(R, r);
+  } else {
+    const uint64_t m2 = (uint64_t)(int64_t)(int32_t)m;
+    const uint64_t xi = __builtin_bit_cast(uint64_t, x), yi = __builtin_bit_cast(uint64_t, y);
+    return __builtin_bit_cast(R, (xi & m2) | (yi & ~m2));
+  }
+}
+
 template <typename R>
 using vec4 = R __attribute__((ext_vector_type(4)));
 template <typename R>
@@ -379,11 +403,13 @@ __device__ inline void adr_apply(const AdrArgs<R> &a, uint32_t n, uint32_t k, co
   R divu = R(0);
 #pragma unroll
   for (int ax = 0; ax < ND; ++ax) {
-    const uint32_t fb = w >> (kWordFace + 4 * ax);
-    const bool o_lo = fb & 1, o_hi = (fb >> 1) & 1, bm1 = (fb >> 2) & 1, bp1 = (fb >> 3) & 1;
-    const uint32_t ff = w >> (kWordFar + 2 * ax);
-    const bool f_lo = ff & 1, f_hi = (ff >> 1) & 1;
+    // face bits as all-ones/zero masks: each select below is one bitwise
+    // select instead of a bit test, a compare and a conditional move
+    const int fb = kWordFace + 4 * ax, ff = kWordFar + 2 * ax;
+    const uint32_t m_lo = bitmask(w, fb), m_hi = bitmask(w, fb + 1);
+    const uint32_t m_bm1 = bitmask(w, fb + 2), m_bp1 = bitmask(w, fb + 3);
     const bool direct = ax == ND - 1;
+    const uint32_t m_flo = direct ? 0u : bitmask(w, ff), m_fhi = direct ? 0u : bitmask(w, ff + 1);
     const R u_lo = g.u_lo[ax], u_hi = g.u_hi[ax];
     divu = (ax == 0) ? R(0) + (u_hi - u_lo) : divu + (u_hi - u_lo);
 #pragma unroll
@@ -393,14 +419,14 @@ __device__ inline void adr_apply(const AdrArgs<R> &a, uint32_t n, uint32_t k, co
       // values at the voxel (Cm1 = C) and a closed far face's at the first
       // neighbour (Cm2 = Cm1); along the others the face bits pick
       const R C = c[j];
-      const R Cm1 = (direct || o_lo) ? cn[j][ax][1] : C;
-      const R Cp1 = (direct || o_hi) ? cn[j][ax][2] : C;
-      const R Cm2 = (direct || f_lo) ? cn[j][ax][0] : Cm1;
-      const R Cp2 = (direct || f_hi) ? cn[j][ax][3] : Cp1;
+      const R Cm1 = direct ? cn[j][ax][1] : bsel(m_lo, cn[j][ax][1], C);
+      const R Cp1 = direct ? cn[j][ax][2] : bsel(m_hi, cn[j][ax][2], C);
+      const R Cm2 = direct ? cn[j][ax][0] : bsel(m_flo, cn[j][ax][0], Cm1);
+      const R Cp2 = direct ? cn[j][ax][3] : bsel(m_fhi, cn[j][ax][3], Cp1);
       // a closed face's difference is +0 and its coefficient is not zeroed:
       // t * +0 = 0 * +0 for a finite t >= 0
-      const R d_lo = bm1 ? t1[j] : t0[j];
-      const R d_hi = bp1 ? t1[j] : t0[j];
+      const R d_lo = bsel(m_bm1, t1[j], t0[j]);
+      const R d_hi = bsel(m_bp1, t1[j], t0[j]);
       const R dif = d_hi * (Cp1 - C) + d_lo * (Cm1 - C);
       tr[j] = (ax == 0) ? R(0) + dif : tr[j] + dif;
       // upwind face values: the upwind side picks the three muscl operands,
@@ -416,10 +442,10 @@ __device__ inline void adr_apply(const AdrArgs<R> &a, uint32_t n, uint32_t k, co
     if (j < ns && a.mobile[j]) tr[j] = tr[j] + c[j] * divu;
 
   // closed faces over 2*nd, edge-replicated neighbours (:1000-1018)
-  int closed = 0;
+  uint32_t open_bits = 0;
 #pragma unroll
-  for (int ax = 0; ax < ND; ++ax)
-    closed += !((w >> (kWordFace + 4 * ax)) & 1) + !((w >> (kWordFace + 4 * ax + 1)) & 1);
+  for (int ax = 0; ax < ND; ++ax) open_bits |= 3u << (kWordFace + 4 * ax);
+  const int closed = 2 * ND - __builtin_popcount(w & open_bits);
 
   // ---- reactions, 3D/tools/prtlb_3d.py:1224-1367
   R r[NSM];
