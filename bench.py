@@ -473,6 +473,72 @@ def features_study(edge=128, cpu_edge=48, buf=10):
     return out
 
 
+def tail_study(counts=(1, 4, 16), edge=48, reps=4):
+    """Latency-bound tail batches: per-launch time of the batched Stokes
+    tick (fluid lists; 50 launches and the per-member sums) and of the
+    batched transport chunk (32 launches), replayed from a captured HIP graph
+    against the same launches issued one by one, for 1, 4 and 16 members of
+    an edge^3 sphere pack (porosity 0.35).  A kernel trace of the same run
+    gives the kernels' own durations, the floor a persistent multi-step
+    launch could approach."""
+    import numpy as np
+    import torch
+    import prtlb
+    from prtlb.batch import AdrBatch, LbmBatch
+    out = {}
+    for n in counts:
+        geoms = [prtlb.sphere_pack((edge, edge, edge), (3.0, 6.0), 0.35, seed=i) for i in range(n)]
+        row = {}
+        for graph in (True, False):
+            b = LbmBatch(19, storage="fp32")
+            b.use_graph = graph
+            for i, g in enumerate(geoms):
+                b.add(i, g, force=1e-6, nit=10 ** 6)
+            for _ in range(2):                      # both launch parities captured
+                b._tick_launches()
+                b.parity += b.every
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                b._tick_launches()
+                b.parity += b.every
+            e1.record()
+            torch.cuda.synchronize()
+            row["stokes_us_per_launch_" + ("graph" if graph else "eager")] = \
+                e0.elapsed_time(e1) * 1e3 / (reps * b.every)
+            row["fluid_nodes"] = int(sum(m.n_fluid for m in b.members.values()))
+            del b
+        for graph in (True, False):
+            a = AdrBatch()
+            a.use_graph = graph
+            for i, g in enumerate(geoms):
+                v = np.zeros((3,) + g.shape, np.float32)
+                v[0] = np.where(g == prtlb.PORE, 1.0, 0.0)
+                a.add(i, g, v, 10.0, 1.0, 3, n_species=2,
+                      chem=prtlb.chemistry.sorption_irreversible(), da_abio=1.0)
+            a._plan()
+            steps = a.GRAPH_STEPS
+            for _ in range(2):
+                a._graph_chunk() if graph else a._launch(a.parity, steps)
+                a.parity += steps
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                a._graph_chunk() if graph else a._launch(a.parity, steps)
+                a.parity += steps
+            e1.record()
+            torch.cuda.synchronize()
+            row["transport_us_per_launch_" + ("graph" if graph else "eager")] = \
+                e0.elapsed_time(e1) * 1e3 / (reps * steps)
+            row["pore_voxels"] = int(sum(m.T.n_pore for m in a.members.values()))
+            del a
+        out["members_%d" % n] = row
+        log("tail %d members: %s" % (n, row))
+    return out
+
+
 def serve_study(n_req=48, rate=4.0, seed=0, budget=200e9, max_steps=100000, transport_cap=None):
     """Serving: Poisson arrivals (rate per s) of solve requests, 30% D2Q9
     (256^2..1024^2 disk packs), 70% D3Q19 (64^3..256^3 sphere packs), Pe and
@@ -518,6 +584,8 @@ def main():
                     help="serving: pore voxels allowed in the transport batch at once")
     ap.add_argument("--adr-max-steps", type=int, default=100000,
                     help="transport step cap of the campaign and serving studies")
+    ap.add_argument("--tail", action="store_true",
+                    help="small batches: graph replay against launches one by one")
     ap.add_argument("--adr-only", action="store_true",
                     help="only the transport step (128^3 and 256^3)")
     ap.add_argument("--fluid-list", action="store_true",
@@ -541,6 +609,9 @@ def main():
     assert torch.cuda.is_available(), "bench needs a HIP device"
     if args.stream_only:
         print(json.dumps({"stream_copy": stream_copy()}), flush=True)
+        return
+    if args.tail:
+        print(json.dumps({"tail": tail_study()}), flush=True)
         return
     if args.adr_only:
         res = {}
