@@ -645,11 +645,19 @@ def main():
         print(json.dumps({"geometry": geometry_study()}), flush=True)
         return
     if args.fluid_list:
+        import numpy as np
         g = prtlb.sphere_pack((128, 128, 128), (4.0, 8.0), 0.35, seed=0)
         res = {}
         for name, st, l16 in (("fp32_link16", "fp32", True), ("fp32_link32", "fp32", False),
                               ("fp64_link16", "fp64", True)):
             res[name] = time_fluid_list(g, st, args.steps, args.warmup, link16=l16)
+        # the same FP32 step on a box with no wall and on a looser pack: the
+        # reads of the two-step wall reversal grow with the wall area
+        res["fp32_link16_no_walls"] = time_fluid_list(
+            np.full((128, 128, 128), prtlb.PORE, np.uint8), "fp32", args.steps, args.warmup)
+        res["fp32_link16_porosity_0.70"] = time_fluid_list(
+            prtlb.sphere_pack((128, 128, 128), (4.0, 8.0), 0.70, seed=0), "fp32", args.steps,
+            args.warmup)
         print(json.dumps({"fluid_list": res}), flush=True)
         return
     if args.campaign:
