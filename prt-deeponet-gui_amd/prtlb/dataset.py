@@ -14,6 +14,11 @@ result under the same group/dataset names.  h5py is not part of this
 build, so the container is an .npz archive whose keys are the HDF5 paths
 ("geom/material", "samples/conc", ...) and whose attributes are stored as
 "<path>.attrs.<name>" arrays; numpy.load(..., allow_pickle=False) reads it.
+
+A 2D campaign (shape (nx, ny): disk packs, D2Q9) is stored as
+3D/tools/build_dataset_2d.py stores it (:896-937): every grid with a
+trailing axis of one voxel (shape attribute (nx, ny, 1)), the dimension
+attribute 2, and velocities with their z component zero.
 """
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -23,17 +28,19 @@ import numpy as np
 from . import campaign as C
 from .adr import adr_host_setup
 from .batch import AdrBatch, LbmBatch
-from .geometry import edt_device, geodesic_device, sphere_pack
+from .geometry import disk_pack, edt_device, geodesic_device, sphere_pack
 from .physics import check_physics_device
 
 PARAM_NAMES = ("pe", "da", "da_abio")
 
 
 def _draw_rock(shape, phi, rng, radius, max_tries=50):
-    """A percolating seeded sphere pack (3D/tools/build_dataset_3d.py:618-632:
-    draw, keep what the inlet reaches, keep only spanning rocks)."""
+    """A percolating seeded sphere pack, or disk pack for a 2D shape
+    (3D/tools/build_dataset_3d.py:618-632, build_dataset_2d.py:642-650: draw,
+    keep what the inlet reaches, keep only spanning rocks)."""
+    pack = disk_pack if len(shape) == 2 else sphere_pack
     for _ in range(max_tries):
-        g = sphere_pack(shape, radius, phi, seed=int(rng.integers(1, 2 ** 31)), connect=True)
+        g = pack(shape, radius, phi, seed=int(rng.integers(1, 2 ** 31)), connect=True)
         if C.percolates(g):
             return g
     raise RuntimeError("no percolating pack at porosity %.3f in %d draws" % (phi, max_tries))
@@ -61,6 +68,9 @@ def build_dataset(n_geom=4, shape=(48, 48, 48), n_sets=2, n_times=5, porosity=(0
     1349-1413)."""
     rng = np.random.default_rng(seed)
     G, S = int(n_geom), int(n_geom) * int(n_sets)
+    nd = len(shape)
+    if nd not in (2, 3):
+        raise ValueError("shape must be (nx, ny) or (nx, ny, nz)")
     t0 = time.perf_counter()
     mats = [_draw_rock(tuple(shape), float(rng.uniform(*porosity)), rng, radius) for _ in range(G)]
     gdf = np.zeros((G,) + tuple(shape), np.float32)
@@ -71,7 +81,7 @@ def build_dataset(n_geom=4, shape=(48, 48, 48), n_sets=2, n_times=5, porosity=(0
     if progress:
         progress("dataset geometry %.1f s" % (time.perf_counter() - t0))
     # ---- Stokes: probe then real, every rock in one fluid-list batch
-    lb = LbmBatch(19, storage=storage)
+    lb = LbmBatch(19 if nd == 3 else 9, storage=storage)
     force = {}
     vel = {}
     for i, g in enumerate(mats):
@@ -148,8 +158,12 @@ def build_dataset(n_geom=4, shape=(48, 48, 48), n_sets=2, n_times=5, porosity=(0
     if n_bad and not keep_complaints:
         raise SystemExit("%d physics complaint(s): this file is not usable as training data, "
                          "and writing it would hide that." % n_bad)
-    ds = write_layout(np.stack(mats), gdf, edt, conc, vels, gi, par, tn_all, settled, dP,
-                      species=list(chem.names)[:ns])
+    mat = np.stack(mats)
+    if nd == 2:
+        # the 2D file's grids carry a trailing axis of one voxel
+        mat, gdf, edt, conc, vels = (a[..., None] for a in (mat, gdf, edt, conc, vels))
+    ds = write_layout(mat, gdf, edt, conc, vels, gi, par, tn_all, settled, dP,
+                      species=list(chem.names)[:ns], dimension=nd)
     # not written: the check_physics complaints per run, and the float32
     # flows and draws the runs used
     ds["_complaints"] = complaints
@@ -158,11 +172,14 @@ def build_dataset(n_geom=4, shape=(48, 48, 48), n_sets=2, n_times=5, porosity=(0
     return ds
 
 
-def write_layout(mat, gdf, edt, conc, vel, gi, par, tn, settled, dP, species, biofilm=None):
+def write_layout(mat, gdf, edt, conc, vel, gi, par, tn, settled, dP, species, biofilm=None,
+                 dimension=None):
     """The reference's file content as a dict: group/dataset paths to arrays,
     attributes as "<path>.attrs.<name>" (3D/tools/build_dataset_3d.py:862-902).
     conc and velocity are stored float16; conc carries the per-chemical
-    scale max over samples and snapshots, floored at 1e-6."""
+    scale max over samples and snapshots, floored at 1e-6.  dimension
+    (default: the grids' rank) is 2 for a 2D campaign stored with a trailing
+    axis of one voxel."""
     S, T, Cn = conc.shape[:3]
     shape = conc.shape[3:]
     scale = np.maximum(conc.reshape(S * T * Cn, -1).max(1).reshape(S, T, Cn).max((0, 1)), 1e-6)
@@ -171,8 +188,8 @@ def write_layout(mat, gdf, edt, conc, vel, gi, par, tn, settled, dP, species, bi
         ".attrs.shape": np.array(shape, np.int32),
         ".attrs.species": np.array([str(s) for s in species]),
         ".attrs.param_names": np.array(PARAM_NAMES),
-        ".attrs.dimension": np.int64(len(shape)),
-        ".attrs.source": np.array("generated_%dd" % len(shape)),
+        ".attrs.dimension": np.int64(len(shape) if dimension is None else dimension),
+        ".attrs.source": np.array("generated_%dd" % (len(shape) if dimension is None else dimension)),
         "geom/gid": np.arange(mat.shape[0], dtype=np.int32),
         "geom/material": np.asarray(mat, np.uint8),
         "geom/gdf": np.asarray(gdf, np.float32),

@@ -287,6 +287,36 @@ def test_dataset_layout_and_content():
     assert "samples/conc" in z.files and z["samples/settled"].dtype == bool
 
 
+def test_dataset_2d_layout_and_content():
+    """A 2D campaign (disk packs, D2Q9) in the 2D builder's file layout
+    (3D/tools/build_dataset_2d.py:896-937): grids with a trailing axis of one
+    voxel, dimension 2, z velocity zero; each stored run equals the same
+    transport solve alone."""
+    from prtlb import dataset as D
+    ds = D.build_dataset(n_geom=2, shape=(40, 32), n_sets=2, n_times=3, radius=(2.0, 4.0),
+                         porosity=(0.4, 0.6), adr_steps=300, seed=5)
+    G, S = 2, 4
+    assert int(ds[".attrs.dimension"]) == 2
+    assert list(ds[".attrs.shape"]) == [40, 32, 1]
+    assert ds["geom/material"].shape == (G, 40, 32, 1)
+    assert ds["geom/gdf"].shape == (G, 40, 32, 1) and ds["geom/edt"].shape == (G, 40, 32, 1)
+    assert ds["samples/conc"].shape == (S, 3, 2, 40, 32, 1)
+    assert ds["samples/velocity"].shape == (S, 3, 40, 32, 1)
+    assert not ds["samples/velocity"][:, 2].any()
+    from oracle import prtlb_oracle as O
+    for i in range(G):
+        ref, _ = O.geodesic(ds["geom/material"][i, ..., 0])
+        assert np.array_equal(ds["geom/gdf"][i, ..., 0], ref)
+    k = 2
+    i, pe, da = ds["_draws"][k]
+    g = ds["geom/material"][i, ..., 0]
+    v = ds["_flows"][i]
+    kw = C._adr_kw(dict(da=da), "sorption_irreversible")
+    out, tn = prtlb.solve_adr(g, v, pe, kw.pop("da"), 3, steps=300, **kw)
+    assert np.array_equal(ds["samples/conc"][k][..., 0], out.astype(np.float16))
+    assert np.array_equal(ds["samples/t_norm"][k], tn)
+
+
 def test_dataset_stops_on_physics_complaint(monkeypatch, capsys):
     """A planted complaint (the host's check returns one for run 1) stops
     the campaign before a dataset exists, with the reference's PHYSICS line
