@@ -376,6 +376,51 @@ int prt_seg_sums(const int32_t *blk0_dev, const int32_t *nblk_dev, int nseg,
                  int width, const double *partials, double *out,
                  void *stream);
 
+/* Launch shapes of the batched steps, the choices of the library's
+ * launch-shape tuner (prtlb.autotune): 2^t_log2 threads per block (64, 128
+ * or 256), 2^e_log2 runs of list entries per thread (1, 2 or 4; thread t of
+ * a block takes entries base + e * threads + t), t_log2 + e_log2 = 8 or 9
+ * (a block covers a tile of 256 or 512 entries), and the block order
+ * (xcd = 1: each of the 8 XCDs runs one contiguous range of blocks; 0:
+ * round robin).  Dense (box) Stokes members take only 256 x 1, either
+ * order; transport partial sums only 256 x 1.  A batch is planned and
+ * stepped with the same shape: a step of another shape than its plan's
+ * updates nothing.  Every shape gives the same bits (the same per-entry
+ * code; partial sums per 256-entry tile in one fixed order).  No
+ * counterpart in the reference. */
+#define PRT_BATCH_SHAPE(t_log2, e_log2, xcd) \
+  ((t_log2) | ((e_log2) << 4) | ((xcd) << 8))
+/* the shapes of prt_lbm_batch_plan/_step and prt_adr_batch_plan/_step */
+#define PRT_BATCH_SHAPE_LBM_DEFAULT PRT_BATCH_SHAPE(8, 0, 1)
+#define PRT_BATCH_SHAPE_ADR_DEFAULT PRT_BATCH_SHAPE(8, 0, 0)
+
+/* Entries per block of a shape, or 0 for a code that is not a shape. */
+int prt_batch_shape_tile(int shape);
+
+/* prt_lbm_batch_plan / prt_adr_batch_plan for a given shape.  *n_tiles
+ * (may be NULL) receives the partial-sum slots a step writes: one per 256
+ * entries of each member, sum over members of ceil(n / 256) (times the
+ * width for transport); blk0 / nblk of prt_seg_sums count these tiles.
+ * At a 256-entry shape n_tiles equals n_blocks. */
+int prt_lbm_batch_plan_shaped(int q, int storage, const prt_lbm_segment *segs,
+                              int nseg, int shape, void *desc_host,
+                              int32_t *blk_seg_host, int64_t blk_capacity,
+                              int64_t *n_blocks, int64_t *n_tiles);
+int prt_adr_batch_plan_shaped(const prt_adr_segment *segs, int nseg, int shape,
+                              void *desc_host, int32_t *blk_seg_host,
+                              int64_t blk_capacity, int64_t *n_blocks,
+                              int64_t *n_tiles, int *max_species);
+
+/* prt_lbm_batch_step / prt_adr_batch_step with the plan's shape; partials
+ * (NULL = none) then hold n_tiles slots (times the width). */
+int prt_lbm_batch_step_shaped(int q, int storage, const void *desc_dev,
+                              const int32_t *blk_seg_dev, int64_t n_blocks,
+                              int parity, int flags, int shape,
+                              double *partials, void *stream);
+int prt_adr_batch_step_shaped(const void *desc_dev, const int32_t *blk_seg_dev,
+                              int64_t n_blocks, int max_species, int parity,
+                              int shape, double *partials, void *stream);
+
 /* ------------------------------------------------------------- geometry */
 
 /* Seeded grain packs on the GPU.  The host draws the candidate grains

@@ -29,6 +29,23 @@ def lbm_tune(bz_log2, by_log2, npt, nt_loads=False, xcd=False):
             | (PRT_LBM_XCD if xcd else 0))
 
 
+def batch_shape(t_log2, e_log2, xcd):
+    """PRT_BATCH_SHAPE: launch shape of a batched step (2^t_log2 threads per
+    block, 2^e_log2 list entries per thread, XCD-contiguous block order)."""
+    return t_log2 | (e_log2 << 4) | (int(bool(xcd)) << 8)
+
+
+BATCH_SHAPE_LBM_DEFAULT = batch_shape(8, 0, 1)
+BATCH_SHAPE_ADR_DEFAULT = batch_shape(8, 0, 0)
+# every shape the batched steps take (prt_batch_shape_tile is non-zero)
+BATCH_SHAPES = [batch_shape(t, e, x) for x in (0, 1)
+                for t, e in ((8, 0), (7, 1), (6, 2), (8, 1), (7, 2))]
+
+
+def batch_shape_name(code):
+    return "%dx%d%s" % (1 << (code & 15), 1 << ((code >> 4) & 15), "x" if code >> 8 else "")
+
+
 class PrtError(RuntimeError):
     """A call into libprtlb_hip.so failed."""
 
@@ -152,6 +169,13 @@ _SIGS = {
     "prt_lbm_batch_step": (_I, [_I, _I, _P, _P, _I64, _I, _I, _P, _P]),
     "prt_adr_batch_step": (_I, [_P, _P, _I64, _I, _I, _P, _P]),
     "prt_seg_sums": (_I, [_P, _P, _I, _I, _P, _P, _P]),
+    "prt_batch_shape_tile": (_I, [_I]),
+    "prt_lbm_batch_plan_shaped": (_I, [_I, _I, _P, _I, _I, _P, _P, _I64, ctypes.POINTER(_I64),
+                                       ctypes.POINTER(_I64)]),
+    "prt_adr_batch_plan_shaped": (_I, [_P, _I, _I, _P, _P, _I64, ctypes.POINTER(_I64),
+                                       ctypes.POINTER(_I64), ctypes.POINTER(ctypes.c_int)]),
+    "prt_lbm_batch_step_shaped": (_I, [_I, _I, _P, _P, _I64, _I, _I, _I, _P, _P]),
+    "prt_adr_batch_step_shaped": (_I, [_P, _P, _I64, _I, _I, _I, _P, _P]),
     "prt_grain_first_cover": (_I, [_I, _I, _I, _P, _P, _I, _P, _P, _I, _P, _P]),
     "prt_grain_histogram": (_I, [_P, _I64, _P, _P]),
     "prt_grain_mask": (_I, [_P, _I64, _I, _I, _I, _P, _P]),

@@ -1,17 +1,26 @@
-"""Launch-shape autotuner for the lattice-Boltzmann step.
+"""Launch-shape autotuners.
 
-Sweeps the block shape (threads along z and y) and the nodes per thread of
-prt_lbm_run on the actual grid, storage and mode, times each candidate with
-HIP events, and keeps the fastest.  Results are cached per (lattice,
-storage, mode, grid) for the process and, if PRTLB_TUNE_CACHE names a file,
-across processes as JSON.
+tune(s): the single-geometry lattice-Boltzmann step.  Sweeps the block shape
+(threads along z and y) and the nodes per thread of prt_lbm_run on the
+actual grid, storage and mode.
+
+tune_batch(b): the batched steps (LbmBatch, AdrBatch).  Sweeps threads per
+block, list entries per thread and the block order (PRT_BATCH_SHAPE) on the
+batch's actual members, restoring their state afterwards.
+
+Both time each candidate with HIP events and keep the fastest.  Results are
+cached for the process and, if PRTLB_TUNE_CACHE names a file, across
+processes as JSON: per (lattice, storage, mode, grid) for tune, per (kind,
+lattice or width, storage, total list entries to half an octave) for
+tune_batch.
 """
 import json
+import math
 import os
 
 import torch
 
-from ._lib import lbm_tune
+from ._lib import BATCH_SHAPES, batch_shape_name, lbm_tune
 
 _CACHE = {}
 
@@ -91,3 +100,84 @@ def tune(s, steps=10, candidates=None, force=False):
     _CACHE[key] = {"tune": best, "ms": best_ms, "table": table, "order": order}
     _save()
     return _CACHE[key]
+
+
+def _batch_key(b):
+    if hasattr(b, "q"):
+        n = sum(m.nodes for m in b.members.values())
+        head = "batch_lbm_q%d_%s_%s" % (b.q, b.storage, "list" if b.sparse else "dense")
+    else:
+        n = sum(m.T.n_pore for m in b.members.values())
+        head = "batch_adr_ns%d" % b.max_species
+    return "%s_n%d" % (head, int(round(2 * math.log2(max(n, 1)))))
+
+
+def _batch_buffers(b):
+    """The device buffers a batched launch writes, member by member."""
+    if hasattr(b, "q"):
+        return [buf for m in b.members.values() for buf in m.f]
+    return [buf for m in b.members.values() for buf in m.T.c]
+
+
+def batch_candidates(b):
+    """The launch shapes batch b can take: dense Stokes members only 256
+    threads with one node each."""
+    if hasattr(b, "q") and not b.sparse:
+        return [c for c in BATCH_SHAPES if (c & 0xFF) == 8]
+    return list(BATCH_SHAPES)
+
+
+def _time_batch(b, steps):
+    """Mean ms per launch of batch b at its planned shape (2 warm-up
+    launches, then `steps`)."""
+    def launches(n):
+        if hasattr(b, "q"):
+            b._launch(b.parity, n, False)
+        else:
+            b._launch(b.parity, n)
+    launches(2)
+    st = torch.cuda.current_stream(b.dev)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    launches(steps)
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def tune_batch(b, steps=6, candidates=None, force=False):
+    """Pick the launch shape of batch b for its current members: every
+    candidate is planned and timed over `steps` launches on the members'
+    own buffers, which are copied before and restored after, so tuning
+    leaves every member's state as it found it (and a shape never changes
+    results).  Sets b.shape; returns True when it differs from the shape of
+    b's current plan (the caller then plans again)."""
+    if not _CACHE:
+        _load()
+    key = _batch_key(b)
+    if key in _CACHE and not force:
+        b.shape = _CACHE[key]["shape"]
+        return b.shape != b.planned_shape
+    bufs = _batch_buffers(b)
+    saved = [t.clone() for t in bufs]
+    auto, b.autotune = b.autotune, False
+    table, best, best_ms = {}, None, None
+    try:
+        for shape in (candidates or batch_candidates(b)):
+            b.shape = shape
+            b._plan()
+            ms = _time_batch(b, steps)
+            table[batch_shape_name(shape)] = ms
+            if best_ms is None or ms < best_ms:
+                best, best_ms = shape, ms
+    finally:
+        b.autotune = auto
+        for t, v in zip(bufs, saved):
+            t.copy_(v)
+        torch.cuda.current_stream(b.dev).synchronize()
+        del saved
+    b.shape = best
+    _CACHE[key] = {"shape": best, "name": batch_shape_name(best), "ms": best_ms, "table": table}
+    _save()
+    return b.shape != b.planned_shape

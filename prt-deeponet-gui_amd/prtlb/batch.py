@@ -14,12 +14,13 @@ The reference solves one geometry per process (3D/tools/build_dataset_3d.py
 replacement for that loop's inner solves.
 """
 import ctypes
+import os
 
 import numpy as np
 import torch
 
-from ._lib import (PRT_LBM_SKIP_SOLID, PRT_LBM_SPARSE, PRT_STORE, AdrSegment,
-                   LbmSegment, PrtError, call, lib)
+from ._lib import (BATCH_SHAPE_ADR_DEFAULT, BATCH_SHAPE_LBM_DEFAULT, PRT_LBM_SKIP_SOLID,
+                   PRT_LBM_SPARSE, PRT_STORE, AdrSegment, LbmSegment, PrtError, call, lib)
 from .adr import AdrRun, TransportSolver
 from .device import device as _device
 from .device import ptr, stream_ptr
@@ -55,6 +56,11 @@ class _Seg:
     pass
 
 
+def _autotune_default():
+    """Batches tune their launch shape unless PRTLB_BATCH_TUNE=0."""
+    return os.environ.get("PRTLB_BATCH_TUNE", "1") != "0"
+
+
 def _width(ns):
     return 1 if ns <= 1 else (2 if ns == 2 else (4 if ns <= 4 else 8))
 
@@ -86,6 +92,10 @@ class LbmBatch:
         self.parity = 0
         self._dirty = True
         self.n_blocks = 0
+        # launch shape (PRT_BATCH_SHAPE); with autotune the tuner picks it
+        # for each new membership size class (prtlb.autotune.tune_batch)
+        self.shape = BATCH_SHAPE_LBM_DEFAULT
+        self.autotune = _autotune_default()
 
     def __len__(self):
         return len(self.members)
@@ -172,34 +182,37 @@ class LbmBatch:
                                  m.link_bits if m.link is not None else 0, 0)
         dsz = lib().prt_lbm_batch_desc_size()
         desc = np.zeros(dsz * len(mem), np.uint8)
-        nb = ctypes.c_int64()
-        call("prt_lbm_batch_plan", self.q, self.st, segs, len(mem), desc.ctypes.data, None, 0,
-             ctypes.byref(nb))
+        nb, nt = ctypes.c_int64(), ctypes.c_int64()
+        call("prt_lbm_batch_plan_shaped", self.q, self.st, segs, len(mem), self.shape,
+             desc.ctypes.data, None, 0, ctypes.byref(nb), ctypes.byref(nt))
         blk = np.zeros(nb.value, np.int32)
-        call("prt_lbm_batch_plan", self.q, self.st, segs, len(mem), desc.ctypes.data,
-             blk.ctypes.data, nb.value, ctypes.byref(nb))
+        call("prt_lbm_batch_plan_shaped", self.q, self.st, segs, len(mem), self.shape,
+             desc.ctypes.data, blk.ctypes.data, nb.value, ctypes.byref(nb), ctypes.byref(nt))
         self.order = mem
         self.desc = torch.from_numpy(desc).to(self.dev)
         self.blk_seg = torch.from_numpy(blk).to(self.dev)
         self.n_blocks = nb.value
-        blk0 = np.zeros(len(mem), np.int32)
-        nblk = np.zeros(len(mem), np.int32)
-        for i in range(1, len(mem)):
-            blk0[i] = blk0[i - 1] + (mem[i - 1].nodes + 255) // 256
-        for i, m in enumerate(mem):
-            nblk[i] = (m.nodes + 255) // 256
+        # partial sums go per 256-node tile of each member, whatever the shape
+        nblk = np.array([(m.nodes + 255) // 256 for m in mem], np.int32)
+        blk0 = np.concatenate([[0], np.cumsum(nblk)[:-1]]).astype(np.int32)
+        assert int(nblk.sum()) == nt.value
         self.blk0 = torch.from_numpy(blk0).to(self.dev)
         self.nblk = torch.from_numpy(nblk).to(self.dev)
-        self.partials = torch.zeros(self.n_blocks, dtype=torch.float64, device=self.dev)
+        self.partials = torch.zeros(nt.value, dtype=torch.float64, device=self.dev)
         self.sums = torch.zeros(len(mem), dtype=torch.float64, device=self.dev)
         self._dirty = False
+        self.planned_shape = self.shape
+        if self.autotune and len(mem):
+            from . import autotune
+            if autotune.tune_batch(self):
+                self._plan()            # the tuner picked another shape: plan for it
 
     def _launch(self, parity, steps, reduce_last):
         s = stream_ptr(self.dev)
         for k in range(steps):
             red = reduce_last and k == steps - 1
-            call("prt_lbm_batch_step", self.q, self.st, ptr(self.desc), ptr(self.blk_seg),
-                 self.n_blocks, (parity + k) & 1, self.flags,
+            call("prt_lbm_batch_step_shaped", self.q, self.st, ptr(self.desc), ptr(self.blk_seg),
+                 self.n_blocks, (parity + k) & 1, self.flags, self.shape,
                  ptr(self.partials) if red else None, s)
 
     def step(self, steps, reduce_last=False):
@@ -296,6 +309,10 @@ class AdrBatch:
         self.members = {}
         self.parity = 0
         self._dirty = True
+        # launch shape (PRT_BATCH_SHAPE); with autotune the tuner picks it
+        # for each new membership size class (prtlb.autotune.tune_batch)
+        self.shape = BATCH_SHAPE_ADR_DEFAULT
+        self.autotune = _autotune_default()
 
     def __len__(self):
         return len(self.members)
@@ -353,8 +370,8 @@ class AdrBatch:
             return
         s = stream_ptr(self.dev)
         for k in range(steps):
-            call("prt_adr_batch_step", ptr(self.desc), ptr(self.blk_seg), self.n_blocks,
-                 self.max_species, (parity + k) & 1, None, s)
+            call("prt_adr_batch_step_shaped", ptr(self.desc), ptr(self.blk_seg), self.n_blocks,
+                 self.max_species, (parity + k) & 1, self.shape, None, s)
 
     def _graph_chunk(self):
         graphs = getattr(self, "_graphs", None)
@@ -378,18 +395,23 @@ class AdrBatch:
                                  T.n_pore, ptr(T.c[0]), ptr(T.c[1]))
         dsz = lib().prt_adr_batch_desc_size()
         desc = np.zeros(dsz * len(mem), np.uint8)
-        nb = ctypes.c_int64()
+        nb, nt = ctypes.c_int64(), ctypes.c_int64()
         nsm = ctypes.c_int()
-        call("prt_adr_batch_plan", segs, len(mem), desc.ctypes.data, None, 0, ctypes.byref(nb),
-             ctypes.byref(nsm))
+        call("prt_adr_batch_plan_shaped", segs, len(mem), self.shape, desc.ctypes.data, None, 0,
+             ctypes.byref(nb), ctypes.byref(nt), ctypes.byref(nsm))
         blk = np.zeros(nb.value, np.int32)
-        call("prt_adr_batch_plan", segs, len(mem), desc.ctypes.data, blk.ctypes.data, nb.value,
-             ctypes.byref(nb), ctypes.byref(nsm))
+        call("prt_adr_batch_plan_shaped", segs, len(mem), self.shape, desc.ctypes.data,
+             blk.ctypes.data, nb.value, ctypes.byref(nb), ctypes.byref(nt), ctypes.byref(nsm))
         self.desc = torch.from_numpy(desc).to(self.dev)
         self.blk_seg = torch.from_numpy(blk).to(self.dev)
         self.n_blocks = nb.value
         self.max_species = nsm.value
         self._dirty = False
+        self.planned_shape = self.shape
+        if self.autotune and self.n_blocks > 0:
+            from . import autotune
+            if autotune.tune_batch(self):
+                self._plan()            # the tuner picked another shape: plan for it
 
     def advance(self):
         """Run every member to the earliest pending event, handle the events.

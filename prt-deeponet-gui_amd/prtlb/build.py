@@ -9,6 +9,7 @@ lattice-Boltzmann kernels keep contraction where their source allows it.
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
@@ -25,6 +26,7 @@ SOURCES = {
     "prt_adr.hip": ["-ffp-contract=off"],
     "prt_reduce.hip": ["-ffp-contract=off"],
     "prt_batch.hip": [],
+    "prt_batch_adr.hip": ["-ffp-contract=off"],
     "prt_geom.hip": ["-ffp-contract=off"],
     "prt_check.hip": ["-ffp-contract=off"],
     "prt_handle.hip": [],
@@ -39,18 +41,27 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False):
+def build(verbose=False, jobs=None):
+    """Compile every source (in parallel, one hipcc per file) and link the
+    library; returns its path."""
     cc = hipcc()
     objdir = os.path.join(HERE, "_obj")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src, extra in SOURCES.items():
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        cmd = [cc, "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + extra
+        cmds.append([cc, "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + extra)
+        objs.append(obj)
+
+    def compile_one(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+
+    jobs = jobs or min(len(cmds), max(1, (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(jobs) as pool:
+        for f in [pool.submit(compile_one, c) for c in cmds]:
+            f.result()
     tmp = OUT + ".tmp"
     cmd = [cc, "-shared", "-o", tmp] + objs + ["--offload-arch=" + ARCH]
     if verbose:

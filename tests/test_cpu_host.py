@@ -356,3 +356,66 @@ def test_sorption_physics_view():
     view, vch = C.physics_view(out, ch, "sorption_irreversible")
     bad = prtlb.check_physics(view, mat, chem=vch, da_abio=1.0)
     assert len(bad) == 1 and "S reaches 1.500, above the 1.000" in bad[0]
+
+
+def test_batch_shapes_and_plans_host_side():
+    """The batched steps' launch shapes: the shape codes the tuner sweeps
+    are the ones the library takes, and a shaped plan lays out blocks of the
+    shape's tile and partial-sum slots per 256 entries (plans only fill host
+    descriptors; the member buffers are never touched here)."""
+    L = _lib.lib()
+    tiles = {_lib.batch_shape(t, e, x): (1 << t) * (1 << e)
+             for x in (0, 1) for t in range(5, 10) for e in range(4)}
+    for code, tile in tiles.items():
+        want = tile if (code in _lib.BATCH_SHAPES) else 0
+        assert L.prt_batch_shape_tile(code) == want, _lib.batch_shape_name(code)
+    assert len(_lib.BATCH_SHAPES) == 10
+    assert {L.prt_batch_shape_tile(c) for c in _lib.BATCH_SHAPES} == {256, 512}
+    assert L.prt_batch_shape_tile(3 << 8 | 8) == 0
+
+    sizes = [1, 255, 256, 257, 700, 1024, 3000]
+    segs = (_lib.LbmSegment * len(sizes))()
+    base = 1 << 20
+    for i, n in enumerate(sizes):
+        a = base + 0x10000 * i
+        segs[i] = _lib.LbmSegment(None, a, a + 0x8000, n, 1, 1, 0, 1.25, 1.6, 1e-6, 1024.0,
+                                  a + 0x4000, n, 16, 0)
+    dsz = L.prt_lbm_batch_desc_size()
+    for code in _lib.BATCH_SHAPES:
+        tile = L.prt_batch_shape_tile(code)
+        desc = np.zeros(dsz * len(sizes), np.uint8)
+        nb, nt = ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("prt_lbm_batch_plan_shaped", 19, 1, segs, len(sizes), code, desc.ctypes.data,
+                  None, 0, ctypes.byref(nb), ctypes.byref(nt))
+        nblk = [(n + tile - 1) // tile for n in sizes]
+        assert nb.value == sum(nblk)
+        assert nt.value == sum((n + 255) // 256 for n in sizes)
+        blk = np.zeros(nb.value, np.int32)
+        _lib.call("prt_lbm_batch_plan_shaped", 19, 1, segs, len(sizes), code, desc.ctypes.data,
+                  blk.ctypes.data, nb.value, ctypes.byref(nb), ctypes.byref(nt))
+        assert np.array_equal(blk, np.repeat(np.arange(len(sizes)), nblk))
+        with pytest.raises(_lib.PrtError):     # the block table too short
+            _lib.call("prt_lbm_batch_plan_shaped", 19, 1, segs, len(sizes), code,
+                      desc.ctypes.data, blk.ctypes.data, nb.value - 1, ctypes.byref(nb),
+                      ctypes.byref(nt))
+    # the unshaped plan is the default shape's
+    nb0 = ctypes.c_int64()
+    _lib.call("prt_lbm_batch_plan", 19, 1, segs, len(sizes), desc.ctypes.data, None, 0,
+              ctypes.byref(nb0))
+    assert nb0.value == sum((n + 255) // 256 for n in sizes)
+    with pytest.raises(_lib.PrtError):
+        _lib.call("prt_lbm_batch_plan_shaped", 19, 1, segs, len(sizes), 7, desc.ctypes.data,
+                  None, 0, ctypes.byref(nb), ctypes.byref(nt))
+    # dense members: 256 threads with one node each, either block order
+    dense = (_lib.LbmSegment * 1)()
+    dense[0] = _lib.LbmSegment(base, base + 0x100, base + 0x200, 4, 4, 4, 0, 1.25, 1.6, 1e-6, 1.0,
+                               None, 0, 0, 0)
+    for code in _lib.BATCH_SHAPES:
+        ok = (code & 0xFF) == 8
+        if ok:
+            _lib.call("prt_lbm_batch_plan_shaped", 19, 1, dense, 1, code, desc.ctypes.data, None,
+                      0, ctypes.byref(nb), ctypes.byref(nt))
+        else:
+            with pytest.raises(_lib.PrtError):
+                _lib.call("prt_lbm_batch_plan_shaped", 19, 1, dense, 1, code, desc.ctypes.data,
+                          None, 0, ctypes.byref(nb), ctypes.byref(nt))
