@@ -332,7 +332,7 @@ def precision_study(cases=((256, 0.25), (256, 0.45), (384, 0.25), (384, 0.45)),
 
 
 def campaign_study(n_geom=64, edge=(48, 160), seed=0, adr_steps=None, max_steps=100000,
-                   adr_precision="fp64"):
+                   adr_precision="fp64", batched_only=False):
     """The batched campaign: n_geom seeded sphere packs (edges U{48..160},
     radii U(3,8), porosity U(0.25,0.5)), Pe log-U(0.1,100), Da log-U(0.01,10),
     irreversible sorption.  Runs it twice: FP32 batched (one launch per
@@ -347,9 +347,19 @@ def campaign_study(n_geom=64, edge=(48, 160), seed=0, adr_steps=None, max_steps=
     t0 = time.perf_counter()
     geoms = {s["key"]: C.make_geometry(s) for s in specs}
     log("campaign geometry %.1f s" % (time.perf_counter() - t0))
+    from prtlb import autotune
     rb, tb = C.run_batched(specs, geoms, storage="fp32", adr_steps=adr_steps, max_steps=max_steps,
                            progress=log)
     log("campaign batched fp32 %r" % tb)
+    # the launch shapes the batches' tuner picked (PRTLB_BATCH_TUNE=0: none)
+    shapes = {k: {"picked": v["name"], "ms": {n: round(t, 4) for n, t in v["table"].items()}}
+              for k, v in autotune._CACHE.items() if k.startswith("batch_")}
+    if batched_only:
+        n = len(specs)
+        return {"n_solves": n, "batched_fp32_s": {k: round(v, 2) for k, v in tb.items()},
+                "batched_solves_per_hour": round(n / tb["total"] * 3600, 1),
+                "batch_tuning": os.environ.get("PRTLB_BATCH_TUNE", "1") != "0",
+                "batch_shapes": shapes}
     rs, ts = C.run_sequential(specs, geoms, storage="fp64", adr_steps=adr_steps,
                               max_steps=max_steps, adr_precision=adr_precision, progress=log)
     log("campaign sequential fp64 %r" % ts)
@@ -372,7 +382,8 @@ def campaign_study(n_geom=64, edge=(48, 160), seed=0, adr_steps=None, max_steps=
             "flow_converged_sequential": sum(1 for k in rs if rs[k]["flow_iters"] < 30000),
             "transport_settled_batched": sum(1 for k in rb if rb[k]["converged"]),
             "transport_settled_sequential": sum(1 for k in rs if rs[k]["converged"]),
-            "adr_steps_mean": float(np.mean([rs[k]["adr_steps"] for k in rs]))}
+            "adr_steps_mean": float(np.mean([rs[k]["adr_steps"] for k in rs])),
+            "batch_shapes": shapes}
 
 
 def geometry_study(edges=(128, 256)):
@@ -577,6 +588,8 @@ def main():
                     help="storage-precision error study (256^3, 384^3)")
     ap.add_argument("--campaign", type=int, default=0,
                     help="batched-campaign study with this many rocks")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="campaign study: the batched schedule only")
     ap.add_argument("--serve", type=int, default=0,
                     help="serving study with this many Poisson requests")
     ap.add_argument("--adr-steps", type=int, default=None)
@@ -662,7 +675,8 @@ def main():
         return
     if args.campaign:
         print(json.dumps({"campaign": campaign_study(args.campaign, adr_steps=args.adr_steps,
-                                                     max_steps=args.adr_max_steps)}),
+                                                     max_steps=args.adr_max_steps,
+                                                     batched_only=args.batched_only)}),
               flush=True)
         return
     if args.serve:

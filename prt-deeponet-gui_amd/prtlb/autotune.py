@@ -102,14 +102,20 @@ def tune(s, steps=10, candidates=None, force=False):
     return _CACHE[key]
 
 
+def batch_entries(b):
+    """List entries one launch of batch b updates (fluid nodes or box nodes
+    of Stokes members, pore voxels of transport members)."""
+    if hasattr(b, "q"):
+        return sum(m.nodes for m in b.members.values())
+    return sum(m.T.n_pore for m in b.members.values())
+
+
 def _batch_key(b):
     if hasattr(b, "q"):
-        n = sum(m.nodes for m in b.members.values())
         head = "batch_lbm_q%d_%s_%s" % (b.q, b.storage, "list" if b.sparse else "dense")
     else:
-        n = sum(m.T.n_pore for m in b.members.values())
         head = "batch_adr_ns%d" % b.max_species
-    return "%s_n%d" % (head, int(round(2 * math.log2(max(n, 1)))))
+    return "%s_n%d" % (head, int(round(math.log2(max(batch_entries(b), 1)))))
 
 
 def _batch_buffers(b):
@@ -146,38 +152,53 @@ def _time_batch(b, steps):
     return e0.elapsed_time(e1) / steps
 
 
-def tune_batch(b, steps=6, candidates=None, force=False):
-    """Pick the launch shape of batch b for its current members: every
+# a shape replaces the default only when it is at least this much faster:
+# differences below it are within the run-to-run spread of a short timing
+BATCH_MARGIN = 0.02
+
+
+def tune_batch(b, steps=8, passes=2, candidates=None, force=False):
+    """Pick the launch shape of batch b for its current members.  Every
     candidate is planned and timed over `steps` launches on the members'
-    own buffers, which are copied before and restored after, so tuning
-    leaves every member's state as it found it (and a shape never changes
-    results).  Sets b.shape; returns True when it differs from the shape of
-    b's current plan (the caller then plans again)."""
+    own buffers, in `passes` interleaved passes (the fastest pass counts),
+    which are copied before and restored after, so tuning leaves every
+    member's state as it found it (and a shape never changes results).  The
+    batch's default shape is kept unless another is BATCH_MARGIN faster.
+    Sets b.shape; returns True when it differs from the shape of b's
+    current plan (the caller then plans again)."""
     if not _CACHE:
         _load()
     key = _batch_key(b)
     if key in _CACHE and not force:
         b.shape = _CACHE[key]["shape"]
         return b.shape != b.planned_shape
+    default = b.default_shape
+    cands = list(candidates or batch_candidates(b))
     bufs = _batch_buffers(b)
     saved = [t.clone() for t in bufs]
     auto, b.autotune = b.autotune, False
-    table, best, best_ms = {}, None, None
+    table = {}
     try:
-        for shape in (candidates or batch_candidates(b)):
-            b.shape = shape
-            b._plan()
-            ms = _time_batch(b, steps)
-            table[batch_shape_name(shape)] = ms
-            if best_ms is None or ms < best_ms:
-                best, best_ms = shape, ms
+        for _ in range(passes):
+            for shape in cands:
+                b.shape = shape
+                b._plan()
+                ms = _time_batch(b, steps)
+                name = batch_shape_name(shape)
+                table[name] = min(ms, table.get(name, ms))
     finally:
         b.autotune = auto
         for t, v in zip(bufs, saved):
             t.copy_(v)
         torch.cuda.current_stream(b.dev).synchronize()
         del saved
+    best = min(cands, key=lambda c: table[batch_shape_name(c)])
+    base = table.get(batch_shape_name(default))
+    if base is not None and table[batch_shape_name(best)] > (1.0 - BATCH_MARGIN) * base:
+        best = default
     b.shape = best
-    _CACHE[key] = {"shape": best, "name": batch_shape_name(best), "ms": best_ms, "table": table}
+    _CACHE[key] = {"shape": best, "name": batch_shape_name(best),
+                   "ms": table[batch_shape_name(best)], "table": table,
+                   "entries": batch_entries(b)}
     _save()
     return b.shape != b.planned_shape

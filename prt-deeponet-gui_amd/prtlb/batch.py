@@ -56,6 +56,25 @@ class _Seg:
     pass
 
 
+class _Tuned:
+    """The launch-shape policy both batch kinds share: the shape is tuned
+    (prtlb.autotune.tune_batch) on the first plan with members and again
+    only when the batch has grown past twice, or shrunk below a quarter of,
+    the entries it was tuned at."""
+
+    def _wants_tuning(self):
+        if not self.autotune or not self.members:
+            return False
+        from .autotune import batch_entries
+        n = batch_entries(self)
+        if n < 1:
+            return False
+        if self._tuned_n is not None and self._tuned_n / 4 <= n <= 2 * self._tuned_n:
+            return False
+        self._tuned_n = n
+        return True
+
+
 def _autotune_default():
     """Batches tune their launch shape unless PRTLB_BATCH_TUNE=0."""
     return os.environ.get("PRTLB_BATCH_TUNE", "1") != "0"
@@ -65,7 +84,7 @@ def _width(ns):
     return 1 if ns <= 1 else (2 if ns == 2 else (4 if ns <= 4 else 8))
 
 
-class LbmBatch:
+class LbmBatch(_Tuned):
     """Stokes members on one lattice (q = 9 or 19), advanced together.
 
     Every member checks FlowConvergence on the same launches (every `every`
@@ -94,8 +113,9 @@ class LbmBatch:
         self.n_blocks = 0
         # launch shape (PRT_BATCH_SHAPE); with autotune the tuner picks it
         # for each new membership size class (prtlb.autotune.tune_batch)
-        self.shape = BATCH_SHAPE_LBM_DEFAULT
+        self.shape = self.default_shape = BATCH_SHAPE_LBM_DEFAULT
         self.autotune = _autotune_default()
+        self._tuned_n = None
 
     def __len__(self):
         return len(self.members)
@@ -202,7 +222,7 @@ class LbmBatch:
         self.sums = torch.zeros(len(mem), dtype=torch.float64, device=self.dev)
         self._dirty = False
         self.planned_shape = self.shape
-        if self.autotune and len(mem):
+        if self._wants_tuning():
             from . import autotune
             if autotune.tune_batch(self):
                 self._plan()            # the tuner picked another shape: plan for it
@@ -300,7 +320,7 @@ class LbmBatch:
         return v32
 
 
-class AdrBatch:
+class AdrBatch(_Tuned):
     """Float32 transport members advanced together, each with its own event
     schedule (snapshot ladder, convergence checks) from AdrRun."""
 
@@ -311,8 +331,9 @@ class AdrBatch:
         self._dirty = True
         # launch shape (PRT_BATCH_SHAPE); with autotune the tuner picks it
         # for each new membership size class (prtlb.autotune.tune_batch)
-        self.shape = BATCH_SHAPE_ADR_DEFAULT
+        self.shape = self.default_shape = BATCH_SHAPE_ADR_DEFAULT
         self.autotune = _autotune_default()
+        self._tuned_n = None
 
     def __len__(self):
         return len(self.members)
@@ -408,7 +429,7 @@ class AdrBatch:
         self.max_species = nsm.value
         self._dirty = False
         self.planned_shape = self.shape
-        if self.autotune and self.n_blocks > 0:
+        if self._wants_tuning():
             from . import autotune
             if autotune.tune_batch(self):
                 self._plan()            # the tuner picked another shape: plan for it
