@@ -589,6 +589,40 @@ __device__ inline bool adr_entry(const AdrArgs<R> &a, const void *tab, uint32_t 
   return true;
 }
 
+// Two list entries k[0], k[1] of an npts-entry list (npts >= 1) in one
+// pass: both records, then every value both stencils read, all requested
+// before either update, so the second entry's loads are in flight while
+// the first is computed (cin and cout are distinct buffers, so no load of
+// one entry depends on a store of the other).  Every load is unconditional:
+// an entry past the list reads the last record, and one that is not a pore
+// voxel reads voxel 0 with zero reach; both are then skipped.  pore[e] says
+// which entries were updated.  The same updates as adr_entry, entry by
+// entry.
+template <typename R, int ND, int NSM, bool NTT>
+__device__ inline void adr_entry2(const AdrArgs<R> &a, const void *tab, uint32_t ta, uint32_t npts,
+                                  const uint32_t (&k)[2], R (&cnew)[2][NSM], bool (&pore)[2]) {
+  VoxGeom<R, ND> g[2];
+  uint32_t n[2], w[2], kk[2];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    kk[e] = min(k[e], npts - 1);
+    tab_head<R, ND, NTT>(tab, kk[e], n[e], w[e], g[e]);
+    tab_faces<R, ND, NTT>(tab, ta, kk[e], g[e]);
+  }
+  R c[2][NSM], cn[2][NSM][ND][4];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    pore[e] = k[e] < npts && w[e] != kPoreWordNone && n[e] < (uint32_t)a.N;
+    const uint32_t ne = pore[e] ? n[e] : 0u, we = pore[e] ? w[e] : 0u;
+    g[e].w = we;
+    geom_index<R, ND>(a, ne, we, g[e]);
+    adr_gather<R, ND, NSM>(a, ne, pore[e] ? kk[e] : 0u, g[e], c[e], cn[e]);
+  }
+#pragma unroll
+  for (int e = 0; e < 2; ++e)
+    if (pore[e]) adr_apply<R, ND, NSM>(a, n[e], kk[e], g[e], c[e], cn[e], cnew[e]);
+}
+
 // Flat voxel index -> (x, y, z).
 template <typename R>
 __device__ inline void adr_coords(const AdrArgs<R> &a, int n, int &x, int &y, int &z) {

@@ -37,9 +37,8 @@ __global__ __launch_bounds__(T, (AdrOcc<NSM, T, E>::min_blocks)) void adr_batch_
   a.cin = newest_a ? d.ca : d.cb;
   a.cout = newest_a ? d.cb : d.ca;
   const uint32_t base = (blk - (uint32_t)d.blk0) * TILE;
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const uint32_t k = base + e * T + threadIdx.x;
+  if constexpr (E == 1) {
+    const uint32_t k = base + threadIdx.x;
     float cnew[NSM];
     bool pore = false;
     if (k < (uint32_t)d.npts)
@@ -47,10 +46,23 @@ __global__ __launch_bounds__(T, (AdrOcc<NSM, T, E>::min_blocks)) void adr_batch_
                        : adr_entry<float, 2, NSM, NTT>(a, d.tab, d.ta, k, cnew);
     if (RED) {
       // pore sums of every chemical (FP64), NSM slots per 64-entry group
-      const int g = (e * T + (int)threadIdx.x) >> 6;
+      const int g = (int)threadIdx.x >> 6;
 #pragma unroll
       for (int j = 0; j < NSM; ++j) group_to_lds((pore && j < a.ns) ? (double)cnew[j] : 0.0, lds, g * NSM + j);
     }
+  } else {
+    // entries in pairs: both entries' loads in flight together
+    auto pair = [&](int e) {
+      const uint32_t k[2] = {base + e * T + threadIdx.x, base + (e + 1) * T + threadIdx.x};
+      if (k[0] < (uint32_t)d.npts) {
+        float cnew[2][NSM];
+        bool pore[2];
+        if (d.nd == 3) adr_entry2<float, 3, NSM, NTT>(a, d.tab, d.ta, (uint32_t)d.npts, k, cnew, pore);
+        else adr_entry2<float, 2, NSM, NTT>(a, d.tab, d.ta, (uint32_t)d.npts, k, cnew, pore);
+      }
+    };
+    pair(0);
+    if constexpr (E == 4) pair(2);
   }
   if (RED)
     tile_partials<TILE>(lds, NSM, base, d.npts, partials + (size_t)(d.rt0 + (blk - d.blk0) * (TILE / 256)) * NSM);
