@@ -1,0 +1,68 @@
+"""The batched-campaign and serving workloads at their full sizes, shortened
+in time only: the 64-rock campaign (edges 48..160) and requests drawn from
+the serving mix (256^2..1024^2 disk packs, 64^3..256^3 sphere packs), each
+checked against the same solves run one at a time / served alone."""
+import numpy as np
+import pytest
+
+from prtlb import campaign as C
+from prtlb import serve
+from prtlb.serve import Request, Server
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.timeout(900)
+def test_campaign_64_rocks_batch_equals_single():
+    """The 64 rocks of the campaign study (the same seeded generator and
+    parameters), flow to 1e-4 and 100 irreversible-sorption steps: one
+    fluid-list batch for every flow and one pore-list batch for every
+    transport run give each rock's flow iterations and transport field bit
+    for bit as its solves one at a time, permeability to 1e-12."""
+    specs = C.campaign_specs(n_geom=64, seed=0)
+    edges = [s["shape"][0] for s in specs]
+    assert len(specs) == 64 and min(edges) < 64 and max(edges) > 150
+    geoms = {s["key"]: C.make_geometry(s) for s in specs}
+    print("box nodes %d, pore nodes %d" % (sum(g.size for g in geoms.values()),
+                                          sum(int((g == 2).sum()) for g in geoms.values())),
+          flush=True)
+    rb, tb = C.run_batched(specs, geoms, storage="fp32", tol=1e-4, adr_steps=100)
+    print("batched", tb, flush=True)
+    rs, ts = C.run_sequential(specs, geoms, storage="fp32", tol=1e-4, adr_steps=100)
+    print("one at a time", ts, flush=True)
+    assert set(rb) == set(rs) == {s["key"] for s in specs}
+    for k in rb:
+        assert rb[k]["flow_iters"] == rs[k]["flow_iters"], k
+        assert rb[k]["permeability"] == pytest.approx(rs[k]["permeability"], rel=1e-12), k
+        assert np.array_equal(rb[k]["out_last"], rs[k]["out_last"]), k
+        assert rb[k]["adr_steps"] == rs[k]["adr_steps"] == 100
+
+
+@pytest.mark.timeout(900)
+def test_serving_mix_at_full_sizes_alone_equals_mix():
+    """Six requests of the serving study's own draw (its seeded Poisson
+    mix): the largest 2D and 3D ones, the smallest 3D one and three more,
+    reversible sorption.  Served together (600 flow iterations each, 300
+    transport steps) every result equals the request served alone."""
+    pool = serve.poisson_requests(32, seed=0)
+    d2 = sorted((r for r in pool if r.g.ndim == 2), key=lambda r: r.g.size)
+    d3 = sorted((r for r in pool if r.g.ndim == 3), key=lambda r: r.g.size)
+    pick = [d2[-1], d3[-1], d3[0], d2[0], d3[len(d3) // 2], d3[len(d3) // 3]]
+    print("sizes", [r.g.shape for r in pick], flush=True)
+    assert max(r.g.shape[0] for r in pick if r.g.ndim == 2) > 900
+    assert max(r.g.shape[0] for r in pick if r.g.ndim == 3) > 230
+
+    def reqs(rs):
+        return [Request(r.key, r.g, r.pe, r.da, arrival=0.0) for r in rs]
+
+    kw = dict(nit=600, tol=0.0, adr_steps=None, max_steps=300)
+    with Server(**kw) as S:
+        mix = S.run(reqs(pick), realtime=False)
+    by_key = {r.key: r for r in mix}
+    for r in pick:
+        with Server(**kw) as S:
+            alone = S.run(reqs([r]), realtime=False)[0]
+        m = by_key[r.key]
+        assert np.array_equal(m.result["velocity"], alone.result["velocity"]), r.g.shape
+        assert np.array_equal(m.result["out"], alone.result["out"]), r.g.shape
+        assert m.result["info"]["steps"] == alone.result["info"]["steps"] == 300
