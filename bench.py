@@ -386,6 +386,70 @@ def campaign_study(n_geom=64, edge=(48, 160), seed=0, adr_steps=None, max_steps=
             "batch_shapes": shapes}
 
 
+def slab_study(E=256, parts=8, steps=50, warmup=5):
+    """One GPU's share of an E^3 solve split along x over `parts` GPUs, at
+    its real shape: the slab step (E/parts own planes between two halo
+    planes, PRT_LBM_SLAB) and its halo pack and unpack kernels, timed with
+    HIP events, beside the whole box's step.  The exchange itself would run
+    over xGMI between GPUs; its time is estimated from the message size at
+    one link's rate."""
+    import torch
+    import prtlb
+    from prtlb.slab import SlabSet
+    from prtlb.stokes import StokesSolver
+    g = prtlb.sphere_pack((E, E, E), (4.0, 8.0), 0.35, seed=0)
+    out = {"grid": E, "parts": parts}
+
+    def timed(fn, n):
+        st = torch.cuda.current_stream()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        for _ in range(warmup):
+            fn()
+        e0.record(st)
+        for _ in range(n):
+            fn()
+        e1.record(st)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    for storage, pop in (("fp64", 8), ("fp32", 4)):
+        T = SlabSet(g, parts, storage=storage)
+        T.exchange(0)
+        own = T.slabs[0]["m"] * E * E
+        ms = timed(lambda: T.step_slab(0, 0, 1), steps)
+        sl = T.slabs[0]
+
+        def halo():
+            s = prtlb.device.stream_ptr(T.dev)
+            f = prtlb.device.ptr(sl["f"][0])
+            for plane, d in ((sl["m"], 1), (1, -1)):
+                prtlb._lib.call("prt_lbm_halo_pack", T.q, T.st, *sl["dims"], f, plane, d,
+                                prtlb.device.ptr(sl["send"][d]), s)
+            for plane, d in ((0, 1), (sl["m"] + 1, -1)):
+                prtlb._lib.call("prt_lbm_halo_unpack", T.q, T.st, *sl["dims"], f, plane, d,
+                                prtlb.device.ptr(sl["send"][d]), s)
+        ms_halo = timed(halo, steps)
+        msg, slab_grid = sl["send"][1].numel() * pop, [sl["dims"][0], E, E]
+        del T, sl
+        torch.cuda.empty_cache()
+        S = StokesSolver(g, storage=storage)
+        ms_box = timed(lambda: S.run(1), steps)
+        del S
+        torch.cuda.empty_cache()
+        gbs = own * 2 * 19 * pop / (ms * 1e-3) / 1e9
+        out[storage] = {"slab_grid": slab_grid, "own_nodes": own,
+                        "slab_step_ms": round(ms, 4), "slab_GBs": round(gbs, 1),
+                        "slab_frac": round(gbs * 1e9 / PEAK_HBM, 4),
+                        "halo_pack_unpack_ms": round(ms_halo, 4),
+                        "message_bytes": msg,
+                        "xgmi_one_message_ms_at_153GBs": round(msg / 153e9 * 1e3, 4),
+                        "box_step_ms": round(ms_box, 4),
+                        "box_step_over_parts_ms": round(ms_box / parts, 4)}
+        log("slab", storage, out[storage])
+    return out
+
+
 def geometry_study(edges=(128, 256)):
     """The geometry and check passes on the GPU against their host forms on
     the same grids: seeded sphere pack + inlet connectivity, biofilm coating,
@@ -607,6 +671,9 @@ def main():
     ap.add_argument("--shapes", action="store_true",
                     help="only the launch-shape tuner on the main grid: every candidate in order "
                          "(2 + 5 iterations each), for counter runs that price each shape")
+    ap.add_argument("--slab", action="store_true",
+                    help="only one GPU's share of the main cube split along x over 8 GPUs: "
+                         "the slab step and its halo kernels")
     ap.add_argument("--geometry", action="store_true",
                     help="only the geometry and check passes, GPU against host")
     ap.add_argument("--features", action="store_true",
@@ -656,6 +723,9 @@ def main():
         return
     if args.geometry:
         print(json.dumps({"geometry": geometry_study()}), flush=True)
+        return
+    if args.slab:
+        print(json.dumps({"slab": slab_study(args.grid)}), flush=True)
         return
     if args.fluid_list:
         import numpy as np

@@ -63,6 +63,11 @@ extern "C" {
 #define PRT_LBM_XCD (1 << 19)
 /* prt_lbm_batch_step: the members are fluid lists (prt_lbm_segment.link) */
 #define PRT_LBM_SPARSE (1 << 1)
+/* prt_lbm_step / _run / _velocity on an x-slab (one GPU's share of a box
+ * split along x): the grid's planes 0 and nx - 1 are halo planes, filled
+ * by the caller before each step (prt_lbm_halo_unpack), and only planes
+ * 1 .. nx - 2 are updated and reduced */
+#define PRT_LBM_SLAB (1 << 20)
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *prt_last_error(void);
@@ -309,6 +314,30 @@ int prt_adr_step_f64(const prt_adr_params *p, int nx, int ny, int nz,
 int prt_adr_run(const prt_adr_params *p, int nx, int ny, int nz,
                 const uint8_t *flags, const float *u, float *c_a, float *c_b,
                 int steps, void *stream);
+
+/* ------------------------------------------------------------- x-slabs */
+
+/* Halo exchange of a box split along x into slabs (the layout one GPU of a
+ * multi-GPU solve holds; no counterpart in the reference, which solves one
+ * box in one process).  A slab of m planes is stepped as the grid
+ * (m + 2, ny, nz) with PRT_LBM_SLAB; before each step its plane 0 must hold
+ * the populations with e_x = +1 of the left neighbour's last own plane and
+ * its plane m + 1 those with e_x = -1 of the right neighbour's first own
+ * plane, from the neighbour's current source buffer.  Each message is
+ * prt_lbm_halo_elems() elements: the populations with e_x = dir (5 for
+ * D3Q19, 3 for D2Q9, in lattice order), each as the plane's ny row runs of
+ * zs elements (the z-row layout's own runs, padding included).
+ * prt_lbm_halo_pack copies them out of plane `plane` of f, unpack writes a
+ * message into plane `plane`.  (nx, ny, nz) is the slab's grid with its
+ * halo planes; dir is +1 or -1.  The message moves between GPUs with
+ * whatever the caller uses (RCCL send/recv, a peer copy). */
+int64_t prt_lbm_halo_elems(int q, int storage, int nx, int ny, int nz);
+int prt_lbm_halo_pack(int q, int storage, int nx, int ny, int nz,
+                      const void *f, int plane, int dir, void *buf,
+                      void *stream);
+int prt_lbm_halo_unpack(int q, int storage, int nx, int ny, int nz,
+                        void *f, int plane, int dir, const void *buf,
+                        void *stream);
 
 /* ------------------------------------------------------------- batching */
 

@@ -43,7 +43,7 @@ __global__ __launch_bounds__(512) void lbm_step_kernel(LbmArgs a) {
   }
   const int z = bx * blockDim.x + threadIdx.x;
   const int y0 = (by * blockDim.y + threadIdx.y) * NPT;
-  const int x = bz;
+  const int x = bz + a.xoff;
   double contrib = 0.0;
 #pragma unroll
   for (int k = 0; k < NPT; ++k) {
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void lbm_velocity_kernel(LbmArgs a, double *v6
   __shared__ double lds[16];
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
-  const int x = blockIdx.z;
+  const int x = blockIdx.z + a.xoff;
   double contrib = 0.0;
   if (z < a.Z && y < a.Y) {
     const int n = (x * a.Y + y) * a.Z + z;
@@ -161,6 +161,37 @@ __global__ void fill_rest_kernel(void *f, long long zs, long long total) {
   }
 }
 
+// Halo message of one x-plane of a slab: the populations with e_x = dir
+// (5 of D3Q19, 3 of D2Q9, in lattice order), each as the plane's Y row runs
+// of zs elements, message element (k*Y + y)*zs + z <-> population element
+// ((plane*Y + y)*Q + i_k)*zs + z.  T is a bit container of the storage
+// width: the copy moves bits.
+template <int Q>
+__device__ inline int halo_pop(int dir, int k) {
+  int seen = 0;
+#pragma unroll
+  for (int i = 0; i < Q; ++i)
+    if (Lat<Q>::cx[i] == dir) {
+      if (seen == k) return i;
+      ++seen;
+    }
+  return 0;
+}
+
+template <int Q, typename T, bool PACK>
+__global__ __launch_bounds__(256) void halo_kernel(T *f, int plane, int dir, int Y, long long zs, T *buf) {
+  constexpr int ND = Q == 19 ? 5 : 3;
+  const long long total = (long long)ND * Y * zs;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < total; m += stride) {
+    const long long z = m % zs, r = m / zs;
+    const int y = (int)(r % Y), k = (int)(r / Y);
+    const long long e = (((long long)plane * Y + y) * Q + halo_pop<Q>(dir, k)) * zs + z;
+    if constexpr (PACK) buf[m] = f[e];
+    else f[e] = buf[m];
+  }
+}
+
 // Deterministic sum: one block, fixed per-thread stride order, fixed combine.
 __global__ __launch_bounds__(1024) void sum_f64_kernel(const double *x, long long n, double *out) {
   __shared__ double part[1024];
@@ -198,6 +229,7 @@ struct Geom {
   int npt;
   bool ntl;
   bool xcd;
+  int xoff;  // PRT_LBM_SLAB: planes 1 .. X-2 are launched, 0 and X-1 are halo
 };
 
 static int make_geom(int q, int nx, int ny, int nz, Geom &g, int flags = 0) {
@@ -245,7 +277,12 @@ static int make_geom(int q, int nx, int ny, int nz, Geom &g, int flags = 0) {
   }
   g.block = dim3(bz, by, 1);
   const int rows = (g.Y + g.npt - 1) / g.npt;
-  g.grid = dim3((g.Z + bz - 1) / bz, (rows + by - 1) / by, g.X);
+  g.xoff = (flags & PRT_LBM_SLAB) ? 1 : 0;
+  if (g.xoff && g.X < 3) {
+    set_error("a slab needs at least one plane between its two halo planes (nx = %d)", g.X);
+    return PRT_E_ARG;
+  }
+  g.grid = dim3((g.Z + bz - 1) / bz, (rows + by - 1) / by, g.X - 2 * g.xoff);
   return PRT_OK;
 }
 
@@ -273,6 +310,7 @@ static void fill_args(LbmArgs &a, int q, const Geom &g, const uint8_t *solid, co
   (void)q;
   a.s16 = (float)s16;
   a.inv_s16 = (float)(1.0 / s16);
+  a.xoff = g.xoff;
 }
 
 template <int Q, int ST, int NPT>
@@ -543,7 +581,7 @@ extern "C" int prt_lbm_velocity(int q, int storage, int nx, int ny, int nz, cons
                                 const void *f_last, const void *f_prev, int flags, double force,
                                 double fp16_scale, double *v64, float *v32, double *partials, void *stream) {
   Geom g;
-  int rc = make_geom(q, nx, ny, nz, g);
+  int rc = make_geom(q, nx, ny, nz, g, flags & PRT_LBM_SLAB);
   if (rc) return rc;
   const bool skip = (flags & PRT_LBM_SKIP_SOLID) != 0;
   if (!solid || !f_last || (skip && (!f_prev || f_prev == f_last)) ||
@@ -575,6 +613,59 @@ extern "C" int prt_lbm_velocity(int q, int storage, int nx, int ny, int nz, cons
   }
 #undef PRT_VK
   return launch_status("prt_lbm_velocity launch");
+}
+
+static int halo_copy(const char *who, bool pack, int q, int storage, int nx, int ny, int nz, void *f, int plane,
+                     int dir, void *buf, void *stream) {
+  Geom g;
+  int rc = make_geom(q, nx, ny, nz, g);
+  if (rc) return rc;
+  if (!f || !buf || storage < 0 || storage > 2 || plane < 0 || plane >= g.X || (dir != 1 && dir != -1)) {
+    set_error("%s: bad arguments (plane %d of %d, dir %d)", who, plane, g.X, dir);
+    return PRT_E_ARG;
+  }
+  const int eb = elem_size(storage);
+  const long long zs = pop_row_pitch(g.Z, eb);
+  const long long total = (long long)(q == 19 ? 5 : 3) * g.Y * zs;
+  const int blocks = (int)((total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096);
+  hipStream_t st = (hipStream_t)stream;
+#define PRT_HK(QQ, TT)                                                                                      \
+  do {                                                                                                      \
+    if (pack) halo_kernel<QQ, TT, true><<<blocks, 256, 0, st>>>((TT *)f, plane, dir, g.Y, zs, (TT *)buf);    \
+    else halo_kernel<QQ, TT, false><<<blocks, 256, 0, st>>>((TT *)f, plane, dir, g.Y, zs, (TT *)buf);        \
+  } while (0)
+  if (q == 19) {
+    if (eb == 8) PRT_HK(19, uint64_t);
+    else if (eb == 4) PRT_HK(19, uint32_t);
+    else PRT_HK(19, uint16_t);
+  } else {
+    if (eb == 8) PRT_HK(9, uint64_t);
+    else if (eb == 4) PRT_HK(9, uint32_t);
+    else PRT_HK(9, uint16_t);
+  }
+#undef PRT_HK
+  return launch_status(who);
+}
+
+extern "C" int64_t prt_lbm_halo_elems(int q, int storage, int nx, int ny, int nz) {
+  Geom g;
+  if (storage < 0 || storage > 2 || make_geom(q, nx, ny, nz, g) != PRT_OK) {
+    set_error("prt_lbm_halo_elems: bad arguments");
+    return -1;
+  }
+  return (int64_t)(q == 19 ? 5 : 3) * g.Y * pop_row_pitch(g.Z, elem_size(storage));
+}
+
+extern "C" int prt_lbm_halo_pack(int q, int storage, int nx, int ny, int nz, const void *f, int plane, int dir,
+                                 void *buf, void *stream) {
+  return halo_copy("prt_lbm_halo_pack", true, q, storage, nx, ny, nz, const_cast<void *>(f), plane, dir, buf,
+                   stream);
+}
+
+extern "C" int prt_lbm_halo_unpack(int q, int storage, int nx, int ny, int nz, void *f, int plane, int dir,
+                                   const void *buf, void *stream) {
+  return halo_copy("prt_lbm_halo_unpack", false, q, storage, nx, ny, nz, f, plane, dir, const_cast<void *>(buf),
+                   stream);
 }
 
 extern "C" int prt_plane_flux(const double *v64, int nx, int ny, int nz, double *plane, void *stream) {

@@ -52,6 +52,7 @@ struct LbmArgs {
   double omp, omm, g, half_g;
   double omh;         // 1 - omm/2, formed on the host as the reference does
   float s16, inv_s16; // FP16 storage scale s and 1/s
+  int xoff;           // first plane a dense launch updates (1 for a slab, else 0)
 };
 
 // Dense grids store populations by z-row: the Q populations of row

@@ -20,6 +20,7 @@ PRT_LBM_SKIP_SOLID = 1
 PRT_LBM_NT_LOADS = 1 << 18
 PRT_LBM_SPARSE = 1 << 1
 PRT_LBM_XCD = 1 << 19
+PRT_LBM_SLAB = 1 << 20
 
 
 def lbm_tune(bz_log2, by_log2, npt, nt_loads=False, xcd=False):
@@ -170,6 +171,9 @@ _SIGS = {
     "prt_adr_batch_step": (_I, [_P, _P, _I64, _I, _I, _P, _P]),
     "prt_seg_sums": (_I, [_P, _P, _I, _I, _P, _P, _P]),
     "prt_batch_shape_tile": (_I, [_I]),
+    "prt_lbm_halo_elems": (_I64, [_I, _I, _I, _I, _I]),
+    "prt_lbm_halo_pack": (_I, [_I, _I, _I, _I, _I, _P, _I, _I, _P, _P]),
+    "prt_lbm_halo_unpack": (_I, [_I, _I, _I, _I, _I, _P, _I, _I, _P, _P]),
     "prt_lbm_batch_plan_shaped": (_I, [_I, _I, _P, _I, _I, _P, _P, _I64, ctypes.POINTER(_I64),
                                        ctypes.POINTER(_I64)]),
     "prt_adr_batch_plan_shaped": (_I, [_P, _I, _I, _P, _P, _I64, ctypes.POINTER(_I64),
