@@ -152,6 +152,9 @@ def _time_batch(b, steps):
     return e0.elapsed_time(e1) / steps
 
 
+# tuning copies the members' buffers: at most this fraction of free memory
+TUNE_MEMORY_FRACTION = 0.5
+
 # a shape replaces the default only when it is at least this much faster:
 # differences below it are within the run-to-run spread of a short timing
 BATCH_MARGIN = 0.02
@@ -175,6 +178,13 @@ def tune_batch(b, steps=8, passes=2, candidates=None, force=False):
     default = b.default_shape
     cands = list(candidates or batch_candidates(b))
     bufs = _batch_buffers(b)
+    # the copy of the members' state must fit beside them: a batch that
+    # fills the device keeps its shape (and is tuned when it is smaller)
+    need = sum(t.numel() * t.element_size() for t in bufs)
+    free, _ = torch.cuda.mem_get_info(b.dev)
+    if need > TUNE_MEMORY_FRACTION * free:
+        b.shape = b.planned_shape
+        return False
     saved = [t.clone() for t in bufs]
     auto, b.autotune = b.autotune, False
     table = {}
