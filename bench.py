@@ -614,7 +614,8 @@ def tail_study(counts=(1, 4, 16), edge=48, reps=4):
     return out
 
 
-def serve_study(n_req=48, rate=4.0, seed=0, budget=200e9, max_steps=100000, transport_cap=None):
+def serve_study(n_req=48, rate=4.0, seed=0, budget=200e9, max_steps=100000, transport_cap=8e6,
+                transport_order="smallest"):
     """Serving: Poisson arrivals (rate per s) of solve requests, 30% D2Q9
     (256^2..1024^2 disk packs), 70% D3Q19 (64^3..256^3 sphere packs), Pe and
     Da log-uniform, reversible sorption; HBM budget admission; continuous
@@ -625,11 +626,12 @@ def serve_study(n_req=48, rate=4.0, seed=0, budget=200e9, max_steps=100000, tran
     reqs = serve.poisson_requests(n_req, rate=rate, seed=seed)
     log("serve geometry %.1f s" % (time.perf_counter() - t0))
     t0 = time.perf_counter()
-    done = serve.Server(hbm_budget=budget, max_steps=max_steps,
-                        transport_cap=transport_cap).run(reqs, realtime=True, progress=log)
+    done = serve.Server(hbm_budget=budget, max_steps=max_steps, transport_cap=transport_cap,
+                        transport_order=transport_order).run(reqs, realtime=True, progress=log)
     wall = time.perf_counter() - t0
     out = serve.latency_summary(done, wall)
     out.update(rate_per_s=rate, hbm_budget_GB=budget / 1e9, transport_cap=transport_cap,
+               transport_order=transport_order,
                n_2d=sum(1 for r in done if r.g.ndim == 2),
                adr_steps_mean=float(sum(r.result["info"]["steps"] for r in done) / len(done)))
     return out
@@ -657,8 +659,10 @@ def main():
     ap.add_argument("--serve", type=int, default=0,
                     help="serving study with this many Poisson requests")
     ap.add_argument("--adr-steps", type=int, default=None)
-    ap.add_argument("--transport-cap", type=float, default=None,
-                    help="serving: pore voxels allowed in the transport batch at once")
+    ap.add_argument("--transport-cap", type=float, default=8e6,
+                    help="serving: pore voxels allowed in the transport batch at once (0: no cap)")
+    ap.add_argument("--transport-order", default="smallest", choices=["arrival", "smallest"],
+                    help="serving: which waiting request enters the transport batch first")
     ap.add_argument("--adr-max-steps", type=int, default=100000,
                     help="transport step cap of the campaign and serving studies")
     ap.add_argument("--tail", action="store_true",
@@ -751,7 +755,8 @@ def main():
         return
     if args.serve:
         print(json.dumps({"serve": serve_study(args.serve, max_steps=args.adr_max_steps,
-                                               transport_cap=args.transport_cap)}),
+                                               transport_cap=args.transport_cap or None,
+                                               transport_order=args.transport_order)}),
               flush=True)
         return
 

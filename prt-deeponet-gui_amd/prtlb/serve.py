@@ -33,23 +33,50 @@ class Request:
     def __init__(self, key, g, pe, da, kinetics="sorption_reversible", arrival=0.0,
                  force=2e-6, probe=False):
         self.key, self.g, self.pe, self.da = key, np.asarray(g), float(pe), float(da)
+        self.n_pore = int((self.g == PORE).sum())
         self.kinetics, self.arrival, self.force, self.probe = kinetics, float(arrival), float(force), probe
         self.t_admit = self.t_done = None
         self.round_admit = self.round_done = None   # scheduling rounds of admission and result
         self.result = None
 
 
+def transport_queue(ready, order, now, max_wait):
+    """The requests waiting for the transport batch, in the order they may
+    enter: "arrival" keeps the order they became ready; "smallest" puts the
+    ones that have waited more than max_wait seconds first (in the order
+    they became ready), then the rest by pore voxels, fewest first."""
+    ready = list(ready)
+    if order == "arrival":
+        return ready
+
+    def key(r):
+        if now - r.t_ready > max_wait:
+            return (0, r.t_ready, 0)
+        return (1, r.n_pore, r.t_ready)
+    return sorted(ready, key=key)
+
+
 class Server:
     """Continuous-batching PRT-LB server on one GPU."""
 
     def __init__(self, hbm_budget=200e9, storage="fp32", tau=0.8, tol=1e-6, nit=30000, n_t=5,
-                 adr_steps=None, max_steps=100000, device=None, transport_cap=None):
+                 adr_steps=None, max_steps=100000, device=None, transport_cap=8e6,
+                 transport_order="smallest", max_wait=60.0):
         # transport_cap: pore voxels allowed in the transport batch at once
-        # (None: no cap).  With a cap, requests beyond it wait first come
-        # first served, so early requests finish earlier (lower median
-        # latency); without one, every admitted request shares the device,
-        # which gives the higher throughput and the lower tail latency.
+        # (None: no cap; a request larger than the cap runs alone).  The
+        # batched step keeps the device busy from about a million pore
+        # voxels on, so a cap of a few million loses no throughput, and the
+        # requests let in finish without waiting for the whole mix.
+        # transport_order: which waiting request enters first, "arrival"
+        # (first come first served) or "smallest" (fewest pore voxels
+        # first: the shortest jobs, since every member runs the same steps
+        # and a step costs it its pore voxels); a request that has waited
+        # max_wait seconds goes ahead of the smaller ones, so none starves.
+        if transport_order not in ("arrival", "smallest"):
+            raise ValueError("transport_order must be 'arrival' or 'smallest'")
         self.transport_cap = transport_cap
+        self.transport_order = transport_order
+        self.max_wait = float(max_wait)
         self.ready = deque()
         self.budget = float(hbm_budget)
         self.storage, self.tau, self.tol, self.nit = storage, tau, tol, nit
@@ -132,6 +159,7 @@ class Server:
                 req.prep = self.pool.submit(adr_host_setup, req.g, v, req.pe, kw["da"],
                                             kw["n_species"], kw["chem"], kw.get("da_abio", 0.0),
                                             "fp32", kw.get("sorption"))
+                req.t_ready = time.perf_counter()
                 self.ready.append(req)
         self._admit_transport()
         if len(self.ab):
@@ -147,9 +175,12 @@ class Server:
                 self.done.append(req)
 
     def _admit_transport(self):
+        if len(self.ready) > 1:
+            self.ready = deque(transport_queue(self.ready, self.transport_order,
+                                               time.perf_counter(), self.max_wait))
         while self.ready:
             req = self.ready[0]
-            n_pore = int((req.g == PORE).sum())
+            n_pore = req.n_pore
             held = sum(m.T.n_pore for m in self.ab.members.values())
             if self.transport_cap and len(self.ab) and held + n_pore > self.transport_cap:
                 break

@@ -203,3 +203,16 @@ def test_force_probe_and_peclet_scaling_by_linearity():
         fp = C.pressure_for_peclet(f, u0, pe, g.shape[0], d_lat)
         u1 = C.axial_mean(flow(fp), g)
         assert abs(C.peclet_from_flow(u1, g.shape[0], d_lat) / pe - 1) < 1e-8
+
+
+def test_transport_queue_order():
+    """The server's transport admission order: first come first served, or
+    fewest pore voxels first with requests past max_wait ahead of all."""
+    from types import SimpleNamespace as NS
+    from prtlb.serve import transport_queue
+    reqs = [NS(key=k, n_pore=n, t_ready=t) for k, n, t in
+            (("a", 500, 1.0), ("b", 100, 2.0), ("c", 300, 3.0), ("d", 100, 4.0), ("e", 900, 5.0))]
+    assert [r.key for r in transport_queue(reqs, "arrival", 10.0, 60.0)] == list("abcde")
+    assert [r.key for r in transport_queue(reqs, "smallest", 10.0, 60.0)] == list("bdcae")
+    # a and b have waited past max_wait: they go first, in ready order
+    assert [r.key for r in transport_queue(reqs, "smallest", 10.0, 7.5)] == list("abdce")
