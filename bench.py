@@ -332,7 +332,7 @@ def precision_study(cases=((256, 0.25), (256, 0.45), (384, 0.25), (384, 0.45)),
 
 
 def campaign_study(n_geom=64, edge=(48, 160), seed=0, adr_steps=None, max_steps=100000,
-                   adr_precision="fp64", batched_only=False):
+                   adr_precision="fp64", batched_only=False, transport_cap=4e6):
     """The batched campaign: n_geom seeded sphere packs (edges U{48..160},
     radii U(3,8), porosity U(0.25,0.5)), Pe log-U(0.1,100), Da log-U(0.01,10),
     irreversible sorption.  Runs it twice: FP32 batched (one launch per
@@ -349,14 +349,15 @@ def campaign_study(n_geom=64, edge=(48, 160), seed=0, adr_steps=None, max_steps=
     log("campaign geometry %.1f s" % (time.perf_counter() - t0))
     from prtlb import autotune
     rb, tb = C.run_batched(specs, geoms, storage="fp32", adr_steps=adr_steps, max_steps=max_steps,
-                           progress=log)
+                           progress=log, transport_cap=transport_cap)
     log("campaign batched fp32 %r" % tb)
     # the launch shapes the batches' tuner picked (PRTLB_BATCH_TUNE=0: none)
     shapes = {k: {"picked": v["name"], "ms": {n: round(t, 4) for n, t in v["table"].items()}}
               for k, v in autotune._CACHE.items() if k.startswith("batch_")}
     if batched_only:
         n = len(specs)
-        return {"n_solves": n, "batched_fp32_s": {k: round(v, 2) for k, v in tb.items()},
+        return {"n_solves": n, "transport_cap": transport_cap,
+                "batched_fp32_s": {k: round(v, 2) for k, v in tb.items()},
                 "batched_solves_per_hour": round(n / tb["total"] * 3600, 1),
                 "batch_tuning": os.environ.get("PRTLB_BATCH_TUNE", "1") != "0",
                 "batch_shapes": shapes}
@@ -374,7 +375,7 @@ def campaign_study(n_geom=64, edge=(48, 160), seed=0, adr_steps=None, max_steps=
                       "batched_solves_per_hour": round(n / tb[st] * 3600, 1),
                       "sequential_solves_per_hour": round(n / ts[st] * 3600, 1),
                       "speedup": round(ts[st] / tb[st], 3)}
-    return {"n_solves": n, "box_nodes": nodes, "pore_nodes": pore,
+    return {"n_solves": n, "box_nodes": nodes, "pore_nodes": pore, "transport_cap": transport_cap,
             "transport_precision_sequential": adr_precision, "adr_max_steps": max_steps,
             "stages": stages,
             "max_perm_rel_err": max(perr), "median_perm_rel_err": float(np.median(perr)),
@@ -614,7 +615,7 @@ def tail_study(counts=(1, 4, 16), edge=48, reps=4):
     return out
 
 
-def serve_study(n_req=48, rate=4.0, seed=0, budget=200e9, max_steps=100000, transport_cap=8e6,
+def serve_study(n_req=48, rate=4.0, seed=0, budget=200e9, max_steps=100000, transport_cap=4e6,
                 transport_order="smallest"):
     """Serving: Poisson arrivals (rate per s) of solve requests, 30% D2Q9
     (256^2..1024^2 disk packs), 70% D3Q19 (64^3..256^3 sphere packs), Pe and
@@ -656,10 +657,12 @@ def main():
                     help="batched-campaign study with this many rocks")
     ap.add_argument("--batched-only", action="store_true",
                     help="campaign study: the batched schedule only")
+    ap.add_argument("--campaign-transport-cap", type=float, default=4e6,
+                    help="campaign study: pore voxels in the transport batch at once (0: no cap)")
     ap.add_argument("--serve", type=int, default=0,
                     help="serving study with this many Poisson requests")
     ap.add_argument("--adr-steps", type=int, default=None)
-    ap.add_argument("--transport-cap", type=float, default=8e6,
+    ap.add_argument("--transport-cap", type=float, default=4e6,
                     help="serving: pore voxels allowed in the transport batch at once (0: no cap)")
     ap.add_argument("--transport-order", default="smallest", choices=["arrival", "smallest"],
                     help="serving: which waiting request enters the transport batch first")
@@ -750,7 +753,9 @@ def main():
     if args.campaign:
         print(json.dumps({"campaign": campaign_study(args.campaign, adr_steps=args.adr_steps,
                                                      max_steps=args.adr_max_steps,
-                                                     batched_only=args.batched_only)}),
+                                                     batched_only=args.batched_only,
+                                                     transport_cap=args.campaign_transport_cap
+                                                     or None)}),
               flush=True)
         return
     if args.serve:

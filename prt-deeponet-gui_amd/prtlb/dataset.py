@@ -28,7 +28,7 @@ import numpy as np
 from . import campaign as C
 from .adr import adr_host_setup
 from .batch import AdrBatch, LbmBatch
-from .geometry import disk_pack, edt_device, geodesic_device, sphere_pack
+from .geometry import PORE, disk_pack, edt_device, geodesic_device, sphere_pack
 from .physics import check_physics_device
 
 PARAM_NAMES = ("pe", "da", "da_abio")
@@ -112,19 +112,27 @@ def build_dataset(n_geom=4, shape=(48, 48, 48), n_sets=2, n_times=5, porosity=(0
         return adr_host_setup(mats[i], vel[i], p, kw["da"], kw["n_species"], kw["chem"],
                               kw.get("da_abio", 0.0), "fp32", kw.get("sorption"))
 
-    # the members' host set-ups run on worker threads while earlier ones join
+    # the members' host set-ups run on worker threads while earlier ones
+    # join; runs join while the batch holds at most C.TRANSPORT_CAP pore
+    # voxels (the batch's table and fields then stay in the Infinity Cache)
+    n_pore = [int((m == PORE).sum()) for m in mats]
+    results = {}
     with ThreadPoolExecutor(max_workers=8) as pool:
         hosts = [pool.submit(prep, dr) for dr in draws]
-        for k, (i, p, d) in enumerate(draws):
-            kw = C._adr_kw(dict(da=d), kinetics)
-            ab.add(k, mats[i], vel[i], p, kw.pop("da"), n_times, steps=adr_steps,
-                   max_steps=max_steps, host=hosts[k].result(), **kw)
-    results = {}
-    while len(ab):
-        for m in ab.advance():
-            info = {}
-            out, tn = ab.finish(m, info)
-            results[m.key] = (out, tn, info)
+        k = 0
+        while k < len(draws) or len(ab):
+            held = sum(m.T.n_pore for m in ab.members.values())
+            while k < len(draws) and not (len(ab) and held + n_pore[draws[k][0]] > C.TRANSPORT_CAP):
+                i, p, d = draws[k]
+                kw = C._adr_kw(dict(da=d), kinetics)
+                ab.add(k, mats[i], vel[i], p, kw.pop("da"), n_times, steps=adr_steps,
+                       max_steps=max_steps, host=hosts[k].result(), **kw)
+                held += n_pore[i]
+                k += 1
+            for m in ab.advance():
+                info = {}
+                out, tn = ab.finish(m, info)
+                results[m.key] = (out, tn, info)
     if progress:
         progress("dataset transport %.1f s" % (time.perf_counter() - t0))
     kw0 = C._adr_kw(dict(da=1.0), kinetics)
