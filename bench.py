@@ -332,7 +332,7 @@ def precision_study(cases=((256, 0.25), (256, 0.45), (384, 0.25), (384, 0.45)),
 
 
 def campaign_study(n_geom=64, edge=(48, 160), seed=0, adr_steps=None, max_steps=100000,
-                   adr_precision="fp64", batched_only=False, transport_cap=4e6):
+                   adr_precision="fp64", batched_only=False, transport_cap=4e6, flow_cap=None):
     """The batched campaign: n_geom seeded sphere packs (edges U{48..160},
     radii U(3,8), porosity U(0.25,0.5)), Pe log-U(0.1,100), Da log-U(0.01,10),
     irreversible sorption.  Runs it twice: FP32 batched (one launch per
@@ -349,14 +349,14 @@ def campaign_study(n_geom=64, edge=(48, 160), seed=0, adr_steps=None, max_steps=
     log("campaign geometry %.1f s" % (time.perf_counter() - t0))
     from prtlb import autotune
     rb, tb = C.run_batched(specs, geoms, storage="fp32", adr_steps=adr_steps, max_steps=max_steps,
-                           progress=log, transport_cap=transport_cap)
+                           progress=log, transport_cap=transport_cap, flow_cap=flow_cap)
     log("campaign batched fp32 %r" % tb)
     # the launch shapes the batches' tuner picked (PRTLB_BATCH_TUNE=0: none)
     shapes = {k: {"picked": v["name"], "ms": {n: round(t, 4) for n, t in v["table"].items()}}
               for k, v in autotune._CACHE.items() if k.startswith("batch_")}
     if batched_only:
         n = len(specs)
-        return {"n_solves": n, "transport_cap": transport_cap,
+        return {"n_solves": n, "transport_cap": transport_cap, "flow_cap": flow_cap,
                 "batched_fp32_s": {k: round(v, 2) for k, v in tb.items()},
                 "batched_solves_per_hour": round(n / tb["total"] * 3600, 1),
                 "batch_tuning": os.environ.get("PRTLB_BATCH_TUNE", "1") != "0",
@@ -657,6 +657,8 @@ def main():
                     help="batched-campaign study with this many rocks")
     ap.add_argument("--batched-only", action="store_true",
                     help="campaign study: the batched schedule only")
+    ap.add_argument("--campaign-flow-cap", type=float, default=0,
+                    help="campaign study: fluid nodes in the Stokes batch at once (0: no cap)")
     ap.add_argument("--campaign-transport-cap", type=float, default=4e6,
                     help="campaign study: pore voxels in the transport batch at once (0: no cap)")
     ap.add_argument("--serve", type=int, default=0,
@@ -755,7 +757,8 @@ def main():
                                                      max_steps=args.adr_max_steps,
                                                      batched_only=args.batched_only,
                                                      transport_cap=args.campaign_transport_cap
-                                                     or None)}),
+                                                     or None,
+                                                     flow_cap=args.campaign_flow_cap or None)}),
               flush=True)
         return
     if args.serve:

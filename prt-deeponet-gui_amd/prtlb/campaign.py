@@ -193,7 +193,7 @@ TRANSPORT_CAP = 4e6
 
 def run_batched(specs, geoms, storage="fp32", tau=0.8, tol=1e-6, nit=30000,
                 kinetics="sorption_irreversible", n_t=5, adr_steps=None, max_steps=100000,
-                hbm_budget=200e9, progress=None, transport_cap=TRANSPORT_CAP):
+                hbm_budget=200e9, progress=None, transport_cap=TRANSPORT_CAP, flow_cap=None):
     """Every Stokes solve in one LbmBatch per lattice, every transport solve
     in one AdrBatch; members admitted while they fit the HBM budget and, for
     transport, while the batch holds at most transport_cap pore voxels (None:
@@ -214,6 +214,9 @@ def run_batched(specs, geoms, storage="fp32", tau=0.8, tol=1e-6, nit=30000,
             g = geoms[key]
             need = lbm_footprint(g.shape, storage, int((g == PORE).sum()) if lb.sparse else None)
             if len(lb) and lb.bytes() + need > hbm_budget:
+                break
+            if flow_cap and len(lb) and (sum(m.nodes for m in lb.members.values())
+                                         + int((g == PORE).sum()) > flow_cap):
                 break
             pending.pop(0)
             force = PROBE_FORCE if stage == "probe" else res[key]["force"]
