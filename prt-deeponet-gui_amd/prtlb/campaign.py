@@ -207,16 +207,17 @@ def run_batched(specs, geoms, storage="fp32", tau=0.8, tol=1e-6, nit=30000,
     lb = LbmBatch(q, storage=storage)
     t_note = time.perf_counter()
     pending = [(sp["key"], "probe") for sp in specs]
+    n_fluid = {sp["key"]: int((geoms[sp["key"]] == PORE).sum()) for sp in specs}
     velocities = {}
     while pending or len(lb):
         while pending:
             key, stage = pending[0]
             g = geoms[key]
-            need = lbm_footprint(g.shape, storage, int((g == PORE).sum()) if lb.sparse else None)
+            need = lbm_footprint(g.shape, storage, n_fluid[key] if lb.sparse else None)
             if len(lb) and lb.bytes() + need > hbm_budget:
                 break
             if flow_cap and len(lb) and (sum(m.nodes for m in lb.members.values())
-                                         + int((g == PORE).sum()) > flow_cap):
+                                         + n_fluid[key] > flow_cap):
                 break
             pending.pop(0)
             force = PROBE_FORCE if stage == "probe" else res[key]["force"]
@@ -245,6 +246,7 @@ def run_batched(specs, geoms, storage="fp32", tau=0.8, tol=1e-6, nit=30000,
     # whole grid) run on worker threads, in order, while members join
     ab = AdrBatch()
     queue = list(specs)
+    n_pore = {sp["key"]: int((geoms[sp["key"]] == PORE).sum()) for sp in specs}
     pool = ThreadPoolExecutor(max_workers=8)
 
     def prep(sp):
@@ -263,7 +265,7 @@ def run_batched(specs, geoms, storage="fp32", tau=0.8, tol=1e-6, nit=30000,
                 if len(ab) and ab.bytes() + adr_footprint(g.shape, kw["n_species"], g.ndim) > hbm_budget:
                     break
                 if transport_cap and len(ab) and (sum(m.T.n_pore for m in ab.members.values())
-                                                  + int((g == PORE).sum()) > transport_cap):
+                                                  + n_pore[sp["key"]] > transport_cap):
                     break
                 queue.pop(0)
                 ab.add(sp["key"], g, velocities[sp["key"]], sp["pe"], kw.pop("da"), n_t,

@@ -114,7 +114,7 @@ class Server:
     def _footprint(self, req):
         g = req.g
         kw = C._adr_kw(dict(da=req.da), req.kinetics)
-        n_fluid = int((g == PORE).sum()) if self.lb[9].sparse else None
+        n_fluid = req.n_pore if self.lb[9].sparse else None
         return max(lbm_footprint(g.shape, self.storage, n_fluid),
                    adr_footprint(g.shape, kw["n_species"], g.ndim))
 
