@@ -451,6 +451,36 @@ def slab_study(E=256, parts=8, steps=50, warmup=5):
     return out
 
 
+def big_study(E=1024, storage="fp32", steps=20):
+    """A box beyond the kernels' 32-bit element index on one GPU: E^3 D3Q19
+    (seeded sphere pack, porosity 0.35) solved as x-slabs of one device
+    (what stokes_d3q19 does for such a grid), halo exchanges included;
+    timed with HIP events over `steps` iterations after 2 warm-up ones."""
+    import torch
+    import prtlb
+    from prtlb.slab import SlabSet
+    t0 = time.perf_counter()
+    g = prtlb.sphere_pack((E, E, E), (4.0, 8.0), 0.35, seed=0)
+    t_geo = time.perf_counter() - t0
+    T = SlabSet(g, None, force=2e-6, storage=storage)
+    T.run(2)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    T.run(steps)
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    pop = {"fp64": 8, "fp32": 4, "fp16": 2}[storage]
+    n = E ** 3
+    gbs = n * 2 * 19 * pop / (ms * 1e-3) / 1e9
+    held = sum(sum(t.numel() * t.element_size() for t in sl["f"]) for sl in T.slabs)
+    return {"grid": E, "storage": storage, "nodes": n, "slabs": len(T.slabs),
+            "ms_per_step": round(ms, 3), "mlups": round(n / (ms * 1e-3) / 1e6, 1),
+            "GBs": round(gbs, 1), "frac": round(gbs * 1e9 / PEAK_HBM, 4),
+            "population_GB": round(held / 1e9, 1), "geometry_s": round(t_geo, 1)}
+
+
 def geometry_study(edges=(128, 256)):
     """The geometry and check passes on the GPU against their host forms on
     the same grids: seeded sphere pack + inlet connectivity, biofilm coating,
@@ -683,6 +713,8 @@ def main():
     ap.add_argument("--slab", action="store_true",
                     help="only one GPU's share of the main cube split along x over 8 GPUs: "
                          "the slab step and its halo kernels")
+    ap.add_argument("--big", type=int, default=0,
+                    help="only an N^3 FP32 D3Q19 box beyond the 32-bit element index, on slabs")
     ap.add_argument("--geometry", action="store_true",
                     help="only the geometry and check passes, GPU against host")
     ap.add_argument("--features", action="store_true",
@@ -732,6 +764,10 @@ def main():
         return
     if args.geometry:
         print(json.dumps({"geometry": geometry_study()}), flush=True)
+        return
+    if args.big:
+        print(json.dumps({"big": big_study(args.big, storage=args.storage if args.storage != "fp64"
+                                           else "fp32")}), flush=True)
         return
     if args.slab:
         print(json.dumps({"slab": slab_study(args.grid)}), flush=True)

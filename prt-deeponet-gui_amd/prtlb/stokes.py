@@ -287,6 +287,12 @@ def _stokes(g, nd, nit, force, tau_lb, magic, tol, info, storage, skip_solid,
             raise ValueError("stokes_d%s needs a %dD grid" % ("2q9" if nd == 2 else "3q19", nd))
     elif np.asarray(g).ndim != nd:
         raise ValueError("stokes_d%s needs a %dD grid" % ("2q9" if nd == 2 else "3q19", nd))
+    shape = tuple(g.shape)
+    if not isinstance(g, torch.Tensor):
+        from .slab import parts_needed
+        if parts_needed(shape, storage) > 1:
+            return _stokes_slabs(np.asarray(g), nd, nit, force, tau_lb, magic, tol, info, storage,
+                                 skip_solid, return_f64, as_tensor, device)
     s = StokesSolver(g, force=force, tau_lb=tau_lb, magic=magic, storage=storage,
                      skip_solid=skip_solid, device=device)
     s.solve(nit=nit, tol=tol, info=info, name="%dD flow" % nd, graph=graph)
@@ -304,6 +310,30 @@ def _stokes(g, nd, nit, force, tau_lb, magic, tol, info, storage, skip_solid,
     if return_f64:
         return out, v64.cpu().numpy()
     return out
+
+
+def _stokes_slabs(g, nd, nit, force, tau_lb, magic, tol, info, storage, skip_solid, return_f64,
+                  as_tensor, device):
+    """A grid whose populations exceed the kernels' 32-bit element index:
+    the same solve on x-slabs of one device (prtlb.slab.SlabSet), the same
+    bits per iteration as one box; float32 output only."""
+    from .slab import SlabSet
+    if return_f64:
+        raise PrtError("return_f64 is not offered for grids beyond the 32-bit element index")
+    S = SlabSet(g, None, force=force, tau_lb=tau_lb, magic=magic, storage=storage,
+                skip_solid=skip_solid, device=device)
+    S.solve(nit=nit, tol=tol, info=info, name="%dD flow" % nd)
+    v32, sux = S.velocity32()
+    solid = torch.from_numpy(np.ascontiguousarray(g != PORE)).to(S.dev)
+    checked_flow(v32, solid, tau_lb, force)
+    del solid
+    if info is not None:
+        info["permeability"] = S.permeability(sux)
+        info["porosity"] = S.n_fluid / S.n
+        info["mean_ux_pore"] = (sux / S.n_fluid) if S.n_fluid else 0.0
+        info["storage"] = storage
+        info["slabs"] = len(S.slabs)
+    return v32 if as_tensor else v32.cpu().numpy()
 
 
 def stokes_d2q9(g, nit=30000, force=2e-6, tau_lb=0.8, magic=3.0 / 16.0, tol=1e-6,

@@ -419,3 +419,23 @@ def test_batch_shapes_and_plans_host_side():
             with pytest.raises(_lib.PrtError):
                 _lib.call("prt_lbm_batch_plan_shaped", 19, 1, dense, 1, code, desc.ctypes.data,
                           None, 0, ctypes.byref(nb), ctypes.byref(nt))
+
+
+def test_slab_count_for_grids_beyond_the_32_bit_index():
+    """Grids whose populations exceed the kernels' 32-bit element index are
+    solved as x-slabs of one device: the fewest slabs that each fit."""
+    from prtlb.slab import parts_needed, slab_dims, split_planes
+    L = _lib.lib()
+    assert parts_needed((256, 256, 256), "fp64") == 1
+    assert parts_needed((2048, 2048), "fp32") == 1
+    for shape, st in (((1024, 1024, 1024), "fp32"), ((768, 768, 768), "fp64"),
+                      ((1100, 900, 1000), "fp16"), ((30000, 30000), "fp64")):
+        code = _lib.PRT_STORE[st]
+        dims = (shape[0], shape[1], 1 if len(shape) == 2 else shape[2])
+        q = 9 if len(shape) == 2 else 19
+        assert L.prt_lbm_pop_elems(q, code, *dims) < 0          # the box is refused
+        p = parts_needed(shape, st)
+        widest = max(x1 - x0 for x0, x1 in split_planes(shape[0], p))
+        assert L.prt_lbm_pop_elems(q, code, *slab_dims(shape, widest)) > 0
+        fewer = -(-shape[0] // (p - 1))
+        assert L.prt_lbm_pop_elems(q, code, *slab_dims(shape, fewer)) < 0

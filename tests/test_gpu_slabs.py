@@ -74,3 +74,22 @@ def test_split_planes_and_halo_messages():
     with pytest.raises(_lib.PrtError):                 # a slab needs an own plane
         _lib.call("prt_lbm_step", 19, 0, 2, 6, 7, ptr(torch.zeros(84, dtype=torch.uint8, device=dev)),
                   ptr(f), ptr(g), 1.25, 1.6, 1e-6, 1.0, _lib.PRT_LBM_SLAB, None, s)
+
+
+@pytest.mark.parametrize("storage,skip", [("fp64", False), ("fp32", True)])
+def test_slab_solve_matches_the_box_solve(storage, skip):
+    """The converging solve on slabs (what the drop-in runs for a grid
+    beyond the 32-bit element index) stops at the box's iteration with the
+    box's field and permeability."""
+    g = prtlb.sphere_pack((40, 20, 20), (3.0, 5.0), 0.6, seed=3)
+    S = StokesSolver(g, force=2e-6, storage=storage, skip_solid=skip)
+    ib, isl = {}, {}
+    S.solve(nit=6000, tol=1e-4, info=ib)
+    vb, _, sb = S.velocity()
+    T = SlabSet(g, 3, force=2e-6, storage=storage, skip_solid=skip)
+    T.solve(nit=6000, tol=1e-4, info=isl)
+    vs, ss = T.velocity32()
+    print("iterations", ib["flow_iters"], isl["flow_iters"], flush=True)
+    assert ib["flow_iters"] == isl["flow_iters"]
+    assert torch.equal(vb, vs)
+    assert T.permeability(ss) == pytest.approx(S.permeability(sb), rel=1e-12)
