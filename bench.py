@@ -646,12 +646,13 @@ def tail_study(counts=(1, 4, 16), edge=48, reps=4):
 
 
 def serve_study(n_req=48, rate=4.0, seed=0, budget=200e9, max_steps=100000, transport_cap=4e6,
-                transport_order="smallest"):
+                transport_order="smallest", baseline=False):
     """Serving: Poisson arrivals (rate per s) of solve requests, 30% D2Q9
     (256^2..1024^2 disk packs), 70% D3Q19 (64^3..256^3 sphere packs), Pe and
     Da log-uniform, reversible sorption; HBM budget admission; continuous
     batching.  Reports p50/p99 time from arrival to converged field and
-    solves per hour."""
+    solves per hour; with baseline, also the same requests solved one at a
+    time in FP64 (serve_sequential)."""
     from prtlb import serve
     t0 = time.perf_counter()
     reqs = serve.poisson_requests(n_req, rate=rate, seed=seed)
@@ -665,7 +666,46 @@ def serve_study(n_req=48, rate=4.0, seed=0, budget=200e9, max_steps=100000, tran
                transport_order=transport_order,
                n_2d=sum(1 for r in done if r.g.ndim == 2),
                adr_steps_mean=float(sum(r.result["info"]["steps"] for r in done) / len(done)))
+    if baseline:
+        out["sequential_fp64"] = serve_sequential(done, max_steps=max_steps,
+                                                  served_per_hour=out["solves_per_hour"])
     return out
+
+
+def serve_sequential(reqs, max_steps=100000, served_per_hour=None):
+    """The served requests solved again one at a time in FP64 on the GPU, in
+    arrival order, with the server's settings (Stokes force, tau 0.8, tol
+    1e-6, 30 000 iterations; transport n_t 5, the same step cap): solves per
+    hour of that schedule against the served rate, and the permeability error
+    of each served FP32 result against its FP64 solve."""
+    import numpy as np
+    import torch
+    import prtlb
+    from prtlb import campaign as C
+    t0 = time.perf_counter()
+    perr, settled, flow_s = [], 0, 0.0
+    for r in sorted(reqs, key=lambda q: q.arrival):
+        fn = prtlb.stokes_d2q9 if r.g.ndim == 2 else prtlb.stokes_d3q19
+        info = {}
+        t1 = time.perf_counter()
+        v = fn(r.g, nit=30000, force=r.force, tau_lb=0.8, tol=1e-6, info=info, storage="fp64")
+        torch.cuda.synchronize()
+        flow_s += time.perf_counter() - t1
+        k64 = info["permeability"]
+        perr.append(abs(r.result["permeability"] - k64) / abs(k64))
+        kw = C._adr_kw(dict(da=r.da), r.kinetics)
+        ainfo = {}
+        prtlb.solve_adr(r.g, v, r.pe, kw.pop("da"), 5, max_steps=max_steps, info=ainfo,
+                        precision="fp64", **kw)
+        settled += bool(ainfo["converged"])
+        log("serve sequential fp64: request %d done (%d steps)" % (r.key, ainfo["steps"]))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per_hour = len(reqs) / wall * 3600.0
+    return dict(wall_s=wall, flow_s=flow_s, solves_per_hour=per_hour,
+                served_over_sequential=(served_per_hour / per_hour) if served_per_hour else None,
+                settled=settled, max_perm_rel_err=float(max(perr)),
+                median_perm_rel_err=float(np.median(perr)))
 
 
 def main():
@@ -693,6 +733,8 @@ def main():
                     help="campaign study: pore voxels in the transport batch at once (0: no cap)")
     ap.add_argument("--serve", type=int, default=0,
                     help="serving study with this many Poisson requests")
+    ap.add_argument("--serve-baseline", action="store_true",
+                    help="serving study: also solve the requests one at a time in FP64")
     ap.add_argument("--adr-steps", type=int, default=None)
     ap.add_argument("--transport-cap", type=float, default=4e6,
                     help="serving: pore voxels allowed in the transport batch at once (0: no cap)")
@@ -800,7 +842,8 @@ def main():
     if args.serve:
         print(json.dumps({"serve": serve_study(args.serve, max_steps=args.adr_max_steps,
                                                transport_cap=args.transport_cap or None,
-                                               transport_order=args.transport_order)}),
+                                               transport_order=args.transport_order,
+                                               baseline=args.serve_baseline)}),
               flush=True)
         return
 
