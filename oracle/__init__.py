@@ -21,6 +21,11 @@ Pinning (how each function is shown to be right, independently of itself):
 * ``stokes_c.stokes`` -- the same iteration in C (oracle/stokes_c.c), for
                          full-size grids: bit-identical to ``stokes`` and
                          held to the goldens (tests/test_oracle_c.py).
+* ``adr_c.stepper``   -- the float32 transport step in C (oracle/adr_c.c), for
+                         the long full-size runs: bit for bit with
+                         ``adr_step`` over every kinetics kind, 2D and 3D
+                         (tests/test_oracle_c.py); planted mistakes (a sign,
+                         swapped face diffusivities) fail it.
 * ``permeability``    -- slit closed form k = (2H^3+H)/(24(H+2)) to 1e-9 for
                          H = 4..20 on the converged oracle flow
                          (tests/test_oracle_c.py).

@@ -717,11 +717,14 @@ def adr_step(S, C):
 def solve_adr(mat, vel, pe, da, n_t, n_species=2, steps=None, tol=2e-3,
               max_steps=100000, info=None, ratio=1.12, chem=None, da_abio=0.0,
               on_progress=None, real=np.float32, return_state=False,
-              sorption=None):
+              sorption=None, stepper=None):
     """Transport solve; 3D/tools/prtlb_3d.py:781-1595.  Returns (out, t_norm)
     and with return_state also the final (ns, *grid) field.  `sorption`
-    (build-defined): dict(da_sorb, da_desorb, c, s) for reversible sorption."""
+    (build-defined): dict(da_sorb, da_desorb, c, s) for reversible sorption.
+    `stepper`: a function of the set-up returning the step to use in place
+    of adr_step (oracle/adr_c.py: the same step computed in C)."""
     S = adr_setup(mat, vel, pe, da, n_species, chem, da_abio, real, sorption)
+    step = adr_step if stepper is None else stepper(S)
     R, ns, mf = S.R, S.ns, S.mf
     shape = S.m.shape
     C = adr_initial(S)
@@ -741,7 +744,7 @@ def solve_adr(mat, vel, pe, da, n_t, n_species=2, steps=None, tol=2e-3,
     limit = int(steps) if steps is not None else max_steps
     while n < limit:
         n += 1
-        C = adr_step(S, C)
+        C = step(S, C)
         if n >= nxt:
             record(n)
             nxt = max(n + 1, int(np.ceil(n * ratio)))

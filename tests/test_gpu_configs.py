@@ -19,6 +19,7 @@ import numpy as np
 import pytest
 import torch
 
+from oracle import adr_c as AC
 from oracle import prtlb_oracle as O
 from oracle import stokes_c as SC
 import prtlb
@@ -93,14 +94,17 @@ def test_config2_stokes_fixed_steps_against_oracle(pack128):
     assert abs(info["permeability"] / k_o - 1) < 1e-10, (info["permeability"], k_o)
 
 
+@pytest.mark.timeout(900)
 def test_config2_converged_flow_and_transport_against_oracle(pack128):
     """The flow run on the GPU under the config's stop rule (1e-8, the
     default 30 000-iteration cap, which this pack reaches first: its mean
     |u_x| still changes by about 3e-6 per 50 iterations there); from that
     developed state 60 more iterations on the GPU and in the oracle agree
-    to 1e-12 (permeability 1e-10).  Then 200 Monod steps on that flow, bit
-    for bit, with the outlet concentration to 1e-10; and the config's full
-    20 000 steps pass the reference's physics checks."""
+    to 1e-12 (permeability 1e-10).  Then 200 Monod steps on that flow
+    against the NumPy oracle, and the config's full 20 000 steps against
+    the oracle with its steps computed in C (oracle/adr_c.c, pinned to the
+    NumPy step bit for bit): snapshots bit for bit, outlet concentration to
+    1e-10, and the reference's physics checks pass."""
     g = pack128
     S = prtlb.StokesSolver(g, force=2e-6, tau_lb=0.8, storage="fp64")
     conv = S.solve(nit=30000, tol=1e-8)
@@ -133,9 +137,14 @@ def test_config2_converged_flow_and_transport_against_oracle(pack128):
     oc, _ = O.outlet_concentration(Co, g)
     assert abs(float(ainfo["outlet_concentration"][0]) - oc[0]) <= 1e-10 * max(abs(oc[0]), 1e-30)
 
-    full = {}
+    full, ofull = {}, {}
     out, tn = prtlb.solve_adr(g, vel, chem=ch, info=full, **dict(kw, steps=20000))
-    assert full["steps"] == 20000
+    outo, tno, Co = O.solve_adr(g, vel, chem=oracle_chem(ch), info=ofull, return_state=True,
+                                stepper=AC.stepper, **dict(kw, steps=20000))
+    assert np.array_equal(out, outo) and np.array_equal(tn, tno)
+    assert full["steps"] == ofull["steps"] == 20000
+    oc, _ = O.outlet_concentration(Co, g)
+    assert abs(float(full["outlet_concentration"][0]) - oc[0]) <= 1e-10 * max(abs(oc[0]), 1e-30)
     assert prtlb.check_physics_device(out, g, chem=ch, da=1.0) == []
     assert float(out.min()) >= 0.0 and float(out.max()) <= 1.0 + 1e-5
 

@@ -3,6 +3,9 @@
 * The C restatement of the FP64 Stokes iteration (oracle/stokes_c.c) equals
   the NumPy oracle bit for bit, stops at the same iteration, and meets the
   reference's golden vectors as the NumPy oracle does.
+* The C restatement of the float32 transport step (oracle/adr_c.c) equals
+  the NumPy oracle's adr_step bit for bit over every kinetics kind, in 2D
+  and 3D, snapshots and final state (float32 words compared).
 * The oracle's Darcy permeability (a quantity the reference does not
   compute) equals the closed form of a slit, k = (2H^3 + H) / (24 (H + 2)),
   for H = 4..20: the node-sampled Poiseuille parabola, which TRT with the
@@ -128,3 +131,59 @@ def test_outlet_concentration_plug_flow(da):
     assert abs(mean[0] - np.exp(-da)) < 0.05
     # uniform velocity over the open rows: the flux-weighted mean is the mean
     assert abs(flux[0] - mean[0]) < 1e-12
+
+
+# ---- the C restatement of the float32 transport step (oracle/adr_c.c)
+from oracle import adr_c as AC  # noqa: E402
+
+
+def _adr_cases():
+    g3 = _spheres((14, 12, 10), 3, phi=0.6, r=2.0)
+    g2 = _disks((36, 22), 4, phi=0.65, r=2.5)
+    gbf = O.coat_with_biofilm(_spheres((12, 11, 9), 5, phi=0.65, r=2.0), 3, 1)[0]
+    nan = np.nan
+    monod = O.Chemistry(["C"], [1.0], [0.0], [1.0], [False], ks_donor=0.5)
+    full = O.Chemistry(["Ac", "A", "P", "Bio"], [1.0, 1.0, 0.0, nan], [0.0, 0.0, 0.0, 0.1],
+                       [1.0, 0.8, 1.0, 0.0], [False, False, False, True], ks_donor=0.3,
+                       ks_acceptor=0.15, alpha=0.4, yield_=0.04, decay_lattice=0.006,
+                       biomass_coupled=True)
+    surface = O.Chemistry(["C"], [1.0], [0.0], [1.0], [False], biotic_on=False,
+                          abiotic_surface_only=True, abiotic_index=0)
+    sorb = O.Chemistry(["C", "S"], [1.0, nan], [0.0, 0.0], [1.0, 0.0], [False, True],
+                       biotic_on=False, abiotic_index=0, abiotic_product_index=1)
+    sorb_surf = O.Chemistry(["C", "S"], [1.0, nan], [0.0, 0.0], [1.0, 0.0], [False, True],
+                            biotic_on=False, abiotic_index=0, abiotic_surface_only=True)
+    dual = O.default_chemistry(2)
+    dual.ks_donor = 0.2
+    biofilm = O.Chemistry(["Ac", "A", "P", "Bio"], [1.0, 1.0, 0.0, nan], [0.0, 0.0, 0.0, 0.1],
+                          [1.0, 1.0, 1.0, 0.0], [False, False, False, True], ks_donor=0.4,
+                          d_biofilm_rel=[0.3, 0.3, 0.3, 0.0], biofilm_code=3, seed_in_biofilm=True,
+                          decay_lattice=0.006, abiotic_product_index=0)
+    return [
+        ("3d-monod", g3, monod, dict(pe=10.0, da=1.0, n_species=1)),
+        ("3d-dual-monod-coupled-abiotic", g3, full, dict(pe=3.0, da=2.0, n_species=4, da_abio=0.5)),
+        ("2d-surface-first-order", g2, surface, dict(pe=40.0, da=0.0, n_species=1, da_abio=1.0)),
+        ("3d-irreversible-sorption", g3, sorb, dict(pe=0.5, da=0.0, n_species=2, da_abio=2.0)),
+        ("2d-reversible-sorption", g2, sorb_surf,
+         dict(pe=5.0, da=0.0, n_species=2, sorption=dict(da_sorb=1.5, da_desorb=0.7, c=0, s=1))),
+        ("2d-dual-monod", g2, dual, dict(pe=20.0, da=3.0, n_species=2)),
+        ("3d-biofilm", gbf, biofilm, dict(pe=8.0, da=1.5, n_species=4, da_abio=0.3)),
+    ]
+
+
+@pytest.mark.parametrize("case", _adr_cases(), ids=lambda c: c[0])
+def test_c_transport_step_equals_numpy_oracle(case):
+    """oracle/adr_c.c against prtlb_oracle.adr_step, every bit of every
+    snapshot (float32 words compared, so -0 against +0 counts too)."""
+    name, g, chem, kw = case
+    fn = O.stokes_d2q9 if g.ndim == 2 else O.stokes_d3q19
+    v = fn(g, nit=150, force=2e-5, tol=0.0)
+    a_info, b_info = {}, {}
+    a = O.solve_adr(g, v, n_t=4, steps=240, chem=chem, info=a_info, return_state=True, **kw)
+    b = O.solve_adr(g, v, n_t=4, steps=240, chem=chem, info=b_info, return_state=True,
+                    stepper=AC.stepper, **kw)
+    for x, y in zip(a, b):
+        assert x.dtype == y.dtype == np.float32
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), name
+    assert a_info == b_info
+    assert float(a[2].max()) > 0.0, name       # something moved in
