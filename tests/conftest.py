@@ -23,3 +23,14 @@ def golden(name):
     d = {k: z[k] for k in z.files if k != "meta"}
     d["meta"] = json.loads(str(z["meta"]))
     return d
+
+
+def oracle_chem(ch):
+    """The oracle's own Chemistry with the attributes of a package one."""
+    from oracle import prtlb_oracle as O
+    return O.Chemistry(ch.names, ch.inlet, ch.c_init, ch.d_rel, ch.immobile, ks_donor=ch.ks_donor,
+                       ks_acceptor=ch.ks_acceptor, alpha=ch.alpha, yield_=ch.yield_,
+                       decay_lattice=ch.decay_lattice, biotic_on=ch.biotic_on,
+                       abiotic_index=ch.abiotic_index,
+                       abiotic_surface_only=ch.abiotic_surface_only,
+                       abiotic_product_index=ch.abiotic_product_index)

@@ -19,6 +19,7 @@ import numpy as np
 import pytest
 import torch
 
+from conftest import oracle_chem
 from oracle import adr_c as AC
 from oracle import prtlb_oracle as O
 from oracle import stokes_c as SC
@@ -36,16 +37,6 @@ def rel(a, b):
 def within_ulp(a, b):
     ulp = np.spacing(np.abs(np.asarray(b, np.float32)))
     return bool((np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)) <= ulp).all())
-
-
-def oracle_chem(ch):
-    """The oracle's own Chemistry with the attributes of a package one."""
-    return O.Chemistry(ch.names, ch.inlet, ch.c_init, ch.d_rel, ch.immobile, ks_donor=ch.ks_donor,
-                       ks_acceptor=ch.ks_acceptor, alpha=ch.alpha, yield_=ch.yield_,
-                       decay_lattice=ch.decay_lattice, biotic_on=ch.biotic_on,
-                       abiotic_index=ch.abiotic_index,
-                       abiotic_surface_only=ch.abiotic_surface_only,
-                       abiotic_product_index=ch.abiotic_product_index)
 
 
 def test_config1_pipeline_against_oracle():

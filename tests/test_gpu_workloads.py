@@ -1,12 +1,19 @@
 """The batched-campaign and serving workloads at their full sizes, shortened
 in time only: the 64-rock campaign (edges 48..160) and requests drawn from
 the serving mix (256^2..1024^2 disk packs, 64^3..256^3 sphere packs), each
-checked against the same solves run one at a time / served alone."""
+checked against the same solves run one at a time / served alone, and the
+transport fields of the largest members against the oracle (its steps
+computed in C, oracle/adr_c.c) bit for bit."""
 import numpy as np
 import pytest
 
+import prtlb
+from conftest import oracle_chem
+from oracle import adr_c as AC
+from oracle import prtlb_oracle as O
 from prtlb import campaign as C
 from prtlb import serve
+from prtlb.batch import AdrBatch
 from prtlb.serve import Request, Server
 
 pytestmark = pytest.mark.gpu
@@ -59,6 +66,18 @@ def test_serving_mix_at_full_sizes_alone_equals_mix():
     with Server(**kw) as S:
         mix = S.run(reqs(pick), realtime=False)
     by_key = {r.key: r for r in mix}
+    # the largest 2D and 3D requests' transport against the oracle, on the
+    # velocity the server handed its transport stage
+    for r in pick[:2]:
+        m = by_key[r.key]
+        akw = C._adr_kw(dict(da=r.da), r.kinetics)
+        info = {}
+        outo, tno = O.solve_adr(r.g, m.result["velocity"], r.pe, akw["da"], 5, n_species=2,
+                                max_steps=300, chem=oracle_chem(akw["chem"]), info=info,
+                                sorption=akw["sorption"], stepper=AC.stepper)
+        print("oracle", r.g.shape, info["steps"], flush=True)
+        assert np.array_equal(m.result["out"], outo), r.g.shape
+        assert np.array_equal(m.result["t_norm"], tno), r.g.shape
     for r in pick:
         with Server(**kw) as S:
             alone = S.run(reqs([r]), realtime=False)[0]
@@ -66,3 +85,37 @@ def test_serving_mix_at_full_sizes_alone_equals_mix():
         assert np.array_equal(m.result["velocity"], alone.result["velocity"]), r.g.shape
         assert np.array_equal(m.result["out"], alone.result["out"]), r.g.shape
         assert m.result["info"]["steps"] == alone.result["info"]["steps"] == 300
+
+
+@pytest.mark.timeout(900)
+def test_campaign_transport_batch_against_oracle():
+    """Rocks of the 64-rock campaign draw -- the three largest (edge near
+    160), the median and the smallest -- with their own Pe and Da,
+    irreversible sorption, 1000 steps, all in one transport batch: every
+    member's snapshots equal the oracle's bit for bit."""
+    specs = C.campaign_specs(n_geom=64, seed=0)
+    by_size = sorted(specs, key=lambda s: s["shape"][0])
+    pick = by_size[-3:] + [by_size[len(by_size) // 2], by_size[0]]
+    print("edges", [s["shape"][0] for s in pick], flush=True)
+    assert max(s["shape"][0] for s in pick) > 150 and min(s["shape"][0] for s in pick) < 56
+    geoms = {s["key"]: C.make_geometry(s) for s in pick}
+    vels = {k: prtlb.stokes_d3q19(g, nit=400, force=2e-6, tol=0.0, storage="fp32")
+            for k, g in geoms.items()}
+    ab = AdrBatch()
+    for sp in pick:
+        kw = C._adr_kw(sp, "sorption_irreversible")
+        ab.add(sp["key"], geoms[sp["key"]], vels[sp["key"]], sp["pe"], kw.pop("da"), 4,
+               steps=1000, **kw)
+    got = {}
+    while len(ab):
+        for m in ab.advance():
+            got[m.key] = ab.finish(m, {})
+    for sp in pick:
+        k = sp["key"]
+        kw = C._adr_kw(sp, "sorption_irreversible")
+        outo, tno = O.solve_adr(geoms[k], vels[k], sp["pe"], kw["da"], 4, n_species=2, steps=1000,
+                                chem=oracle_chem(kw["chem"]), da_abio=kw["da_abio"],
+                                stepper=AC.stepper)
+        print("rock", k, geoms[k].shape, flush=True)
+        assert np.array_equal(got[k][0], outo), k
+        assert np.array_equal(got[k][1], tno), k
