@@ -227,6 +227,36 @@ __device__ inline void gather_sparse(const LbmArgs &a, const L *__restrict__ lin
   }
 }
 
+// FP16 storage rounds stochastically: the 13 bits of float32 significand
+// that FP16 drops get a dither added before they are cut, so an increment
+// smaller than half an FP16 unit still moves the stored value with the
+// matching probability instead of being rounded away at every step (with
+// round-to-nearest the shifted populations of slow pore throats stall).
+// The dither comes from the node's own post-collision populations (the low
+// significand bits of the rest population, or of population 1 for the rest
+// population itself), mixed per direction: the same state gives the same
+// bits, with no counter or generator state, so every launch path agrees.
+//
+// Odd multipliers, one per direction, that turn the node's seed into each
+// population's dither (the high 13 bits of the product).
+constexpr uint32_t kDitherMul[19] = {
+    0x9E3779B1u, 0x85EBCA77u, 0xC2B2AE3Du, 0x27D4EB2Fu, 0x165667B1u, 0xD3A2646Bu, 0xFD7046C5u,
+    0xB55A4F09u, 0x8F1BBCDDu, 0xCA62C1D7u, 0x5A827999u, 0x6ED9EBA1u, 0x3C6EF373u, 0xA54FF53Bu,
+    0x510E527Fu, 0x9B05688Du, 0x1F83D9ABu, 0x5BE0CD19u, 0xCBBB9D5Du};
+template <int Q>
+__device__ inline uint32_t f16_dither_seed(const float *fp, int i) {
+  return __float_as_uint(fp[i == 0 ? 1 : 0]);
+}
+// x rounded to FP16 with the dither: the dither is added to the 13 bits
+// FP16 drops and they are cut, which leaves at most FP16's 11 significand
+// bits, so the conversion is exact in FP16's normal range.  Infinities and
+// quiet NaNs keep their bits (their low 13 bits are zero, and the add does
+// not reach the exponent).
+__device__ inline unsigned short f16_round_dithered(float x, uint32_t seed, int i) {
+  const uint32_t d = (seed * kDitherMul[i]) >> 19;
+  return __half_as_ushort(__float2half_rn(__uint_as_float((__float_as_uint(x) + d) & ~0x1fffu)));
+}
+
 // Store the Q post-collision populations: population i at element
 // base + i * stride (dense: the node's own element and zs; fluid list:
 // list_elem(k, 0, Q) and kListTile).
@@ -237,7 +267,7 @@ __device__ inline void scatter(const LbmArgs &a, size_t base, long long stride, 
   for (int i = 0; i < Q; ++i) {
     if constexpr (ST == PRT_STORE_FP16) {
       unsigned short *dst = static_cast<unsigned short *>(a.dst);
-      const unsigned short v = __half_as_ushort(__float2half_rn(fp[i] * st16<Q>(a, i)));
+      const unsigned short v = f16_round_dithered(fp[i] * st16<Q>(a, i), f16_dither_seed<Q>(fp, i), i);
       __builtin_nontemporal_store(v, dst + base + (size_t)i * stride);
     } else {
       T *dst = static_cast<T *>(a.dst);

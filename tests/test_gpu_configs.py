@@ -141,14 +141,15 @@ def test_config2_converged_flow_and_transport_against_oracle(pack128):
 
 
 # Config 3: storage precision against the FP64 GPU solve after a fixed
-# iteration count.  Bounds: the errors measured on an MI355X (largest of the
-# three cases: FP32 velocity 1.6e-5, permeability 1.8e-6, concentration
-# 5.7e-6; FP16 2.5e-2, 5.5e-4, 1.8e-3) with headroom.  FP16 storage misses
-# the 1e-4 permeability bound at porosity 0.25; FP32 meets it with room.
+# iteration count.  Bounds: the errors measured on an MI355X with headroom
+# (FP32: velocity 1.6e-5, permeability 1.8e-6, concentration 5.7e-6; FP16
+# with its dithered rounding: velocity 9.7e-3, permeability 9.1e-7 at 256^3).
+# Both storages keep the permeability inside the campaign's 1e-4 bound; with
+# round-to-nearest FP16 missed it at porosity 0.25 (4.2e-4).
 PREC_ITERS = 6000
 PREC_BOUNDS = {  # storage: (velocity, permeability, concentration)
     "fp32": (1e-4, 2e-6, 1e-5),
-    "fp16": (5e-2, 1e-3, 3e-3),
+    "fp16": (5e-2, 1e-4, 3e-3),
 }
 
 
