@@ -47,7 +47,8 @@ extern "C" {
 /* population storage (all collide arithmetic in FP64 / FP32 / FP32) */
 #define PRT_STORE_FP64 0 /* f_i, double: the reference's own arithmetic */
 #define PRT_STORE_FP32 1 /* f_i - w_i, float */
-#define PRT_STORE_FP16 2 /* (f_i - w_i) * (s / w_i), IEEE half */
+#define PRT_STORE_FP16 2 /* (f_i - w_i) * (s / w_i), IEEE half, stored with a
+                            deterministic dither (rounded stochastically) */
 
 /* prt_lbm_step / prt_lbm_run flags */
 #define PRT_LBM_SKIP_SOLID 1 /* solid nodes are neither read nor written */
