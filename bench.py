@@ -332,7 +332,8 @@ def precision_study(cases=((256, 0.25), (256, 0.45), (384, 0.25), (384, 0.45)),
 
 
 def campaign_study(n_geom=64, edge=(48, 160), seed=0, adr_steps=None, max_steps=100000,
-                   adr_precision="fp64", batched_only=False, transport_cap=4e6, flow_cap=None):
+                   adr_precision="fp64", batched_only=False, transport_cap=4e6, flow_cap=None,
+                   storage="fp32"):
     """The batched campaign: n_geom seeded sphere packs (edges U{48..160},
     radii U(3,8), porosity U(0.25,0.5)), Pe log-U(0.1,100), Da log-U(0.01,10),
     irreversible sorption.  Runs it twice: FP32 batched (one launch per
@@ -348,9 +349,9 @@ def campaign_study(n_geom=64, edge=(48, 160), seed=0, adr_steps=None, max_steps=
     geoms = {s["key"]: C.make_geometry(s) for s in specs}
     log("campaign geometry %.1f s" % (time.perf_counter() - t0))
     from prtlb import autotune
-    rb, tb = C.run_batched(specs, geoms, storage="fp32", adr_steps=adr_steps, max_steps=max_steps,
+    rb, tb = C.run_batched(specs, geoms, storage=storage, adr_steps=adr_steps, max_steps=max_steps,
                            progress=log, transport_cap=transport_cap, flow_cap=flow_cap)
-    log("campaign batched fp32 %r" % tb)
+    log("campaign batched %s %r" % (storage, tb))
     # the launch shapes the batches' tuner picked (PRTLB_BATCH_TUNE=0: none)
     shapes = {k: {"picked": v["name"], "ms": {n: round(t, 4) for n, t in v["table"].items()}}
               for k, v in autotune._CACHE.items() if k.startswith("batch_")}
@@ -376,6 +377,7 @@ def campaign_study(n_geom=64, edge=(48, 160), seed=0, adr_steps=None, max_steps=
                       "sequential_solves_per_hour": round(n / ts[st] * 3600, 1),
                       "speedup": round(ts[st] / tb[st], 3)}
     return {"n_solves": n, "box_nodes": nodes, "pore_nodes": pore, "transport_cap": transport_cap,
+            "batched_storage": storage,
             "transport_precision_sequential": adr_precision, "adr_max_steps": max_steps,
             "stages": stages,
             "max_perm_rel_err": max(perr), "median_perm_rel_err": float(np.median(perr)),
@@ -731,6 +733,8 @@ def main():
                     help="campaign study: fluid nodes in the Stokes batch at once (0: no cap)")
     ap.add_argument("--campaign-transport-cap", type=float, default=4e6,
                     help="campaign study: pore voxels in the transport batch at once (0: no cap)")
+    ap.add_argument("--campaign-storage", default="fp32", choices=["fp32", "fp16"],
+                    help="campaign study: population storage of the batched Stokes stage")
     ap.add_argument("--serve", type=int, default=0,
                     help="serving study with this many Poisson requests")
     ap.add_argument("--serve-baseline", action="store_true",
@@ -836,7 +840,8 @@ def main():
                                                      batched_only=args.batched_only,
                                                      transport_cap=args.campaign_transport_cap
                                                      or None,
-                                                     flow_cap=args.campaign_flow_cap or None)}),
+                                                     flow_cap=args.campaign_flow_cap or None,
+                                                     storage=args.campaign_storage)}),
               flush=True)
         return
     if args.serve:
