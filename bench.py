@@ -236,6 +236,29 @@ def _oracle_stokes_worker(args):
     return time.perf_counter() - t0
 
 
+def cpu_campaign_estimate(campaign, cpu):
+    """Converged solves per hour of the 64-rock campaign on this host, one
+    process per core, from the campaign's own work (every Stokes node
+    iteration of the probe and real solves over the dense box, every
+    transport pore-voxel step) at the host rates measured here: the
+    whole-host D3Q19 rate and the one-core sorption transport rate times the
+    processes.  An estimate from measured rates, not a timed campaign (that
+    would take the host about the time printed)."""
+    w = campaign["work"]
+    procs = cpu["all_cores_procs"]
+    stokes_s = w["stokes_node_iterations"] / (cpu["d3q19_64cube_mlups_all_cores"] * 1e6)
+    adr_s = w["transport_pore_voxel_steps"] / (
+        cpu["adr_3d_64cube_sorption_pore_voxel_steps_per_s_1core"] * procs)
+    total = stokes_s + adr_s
+    gpu = campaign["stages"]["total"]
+    return {"kind": "estimate from measured host rates", "procs": procs,
+            "stokes_s": round(stokes_s, 1), "transport_s": round(adr_s, 1),
+            "total_h": round(total / 3600, 2),
+            "solves_per_hour": round(campaign["n_solves"] / total * 3600, 2),
+            "gpu_batched_over_host": round(gpu["batched_solves_per_hour"] * total
+                                           / (campaign["n_solves"] * 3600), 1)}
+
+
 def cpu_extras(procs=None):
     """The other host figures of the CPU comparison, each a few seconds:
     the NumPy oracle's D2Q9 step (128^2 disk pack, 200 iterations), its
@@ -270,6 +293,17 @@ def cpu_extras(procs=None):
                                  immobile=[False], ks_donor=0.5, biotic_on=True))
     dt = time.perf_counter() - t0
     out["adr_3d_64cube_pore_voxel_steps_per_s_1core"] = round(int((g3 == 2).sum()) * 200 / dt, 1)
+    # the campaign's kinetics: irreversible sorption (a mobile and an
+    # immobile chemical, abiotic rate on the surface)
+    t0 = time.perf_counter()
+    O.solve_adr(g3, v3, 10.0, 0.0, 3, n_species=2, steps=200, da_abio=1.0,
+                chem=O.Chemistry(names=["C", "S"], inlet=[1.0, np.nan], c_init=[0.0, 0.0],
+                                 d_rel=[1.0, 0.0], immobile=[False, True], biotic_on=False,
+                                 abiotic_index=0, abiotic_product_index=1,
+                                 abiotic_surface_only=True))
+    dt = time.perf_counter() - t0
+    out["adr_3d_64cube_sorption_pore_voxel_steps_per_s_1core"] = round(int((g3 == 2).sum()) * 200
+                                                                       / dt, 1)
     procs = procs or min(16, host_cpu()["usable_cpus"] or 1)
     ctx = mp.get_context("spawn")            # fresh interpreters: no GPU state inherited
     t0 = time.perf_counter()
@@ -376,8 +410,14 @@ def campaign_study(n_geom=64, edge=(48, 160), seed=0, adr_steps=None, max_steps=
                       "batched_solves_per_hour": round(n / tb[st] * 3600, 1),
                       "sequential_solves_per_hour": round(n / ts[st] * 3600, 1),
                       "speedup": round(ts[st] / tb[st], 3)}
+    # the sequential schedule's work, for the host estimate (cpu_campaign_estimate)
+    work = {"stokes_node_iterations": sum(
+                (rs[s["key"]]["probe_iters"] + rs[s["key"]]["flow_iters"]) * int(np.prod(s["shape"]))
+                for s in specs),
+            "transport_pore_voxel_steps": sum(
+                rs[s["key"]]["adr_steps"] * int((geoms[s["key"]] == 2).sum()) for s in specs)}
     return {"n_solves": n, "box_nodes": nodes, "pore_nodes": pore, "transport_cap": transport_cap,
-            "batched_storage": storage,
+            "batched_storage": storage, "work": work,
             "transport_precision_sequential": adr_precision, "adr_max_steps": max_steps,
             "stages": stages,
             "max_perm_rel_err": max(perr), "median_perm_rel_err": float(np.median(perr)),
@@ -905,6 +945,9 @@ def main():
         if not args.no_extras:
             log("cpu extras ...")
             extras["cpu_extras"] = cpu_extras()
+            if "campaign_64" in extras:
+                extras["cpu_extras"]["campaign_64_host"] = cpu_campaign_estimate(
+                    extras["campaign_64"], extras["cpu_extras"])
     extras["bench_wall_s"] = round(time.perf_counter() - t_start, 1)
     line = {
         "metric": "D3Q19 TRT MLUPS & %HBM peak; batched solves/hr + p99 latency at fixed error",

@@ -157,13 +157,14 @@ def run_sequential(specs, geoms, storage="fp64", tau=0.8, tol=1e-6, nit=30000,
     for sp in specs:
         g = geoms[sp["key"]]
         fn = stokes_d2q9 if g.ndim == 2 else stokes_d3q19
-        probe = fn(g, nit=nit, force=PROBE_FORCE, tau_lb=tau, tol=tol, storage=storage)
+        pinfo = {}
+        probe = fn(g, nit=nit, force=PROBE_FORCE, tau_lb=tau, tol=tol, info=pinfo, storage=storage)
         force = lowered_force(probe, 2e-6)
         info = {}
         flows[sp["key"]] = fn(g, nit=nit, force=force, tau_lb=tau, tol=tol, info=info,
                               storage=storage)
         res[sp["key"]] = dict(permeability=info["permeability"], flow_iters=info["flow_iters"],
-                              force=force)
+                              probe_iters=pinfo["flow_iters"], force=force)
         if progress:
             progress("flow %s done (%d its)" % (sp["key"], info["flow_iters"]))
     torch.cuda.synchronize()
@@ -233,6 +234,7 @@ def run_batched(specs, geoms, storage="fp32", tau=0.8, tol=1e-6, nit=30000,
             v = lb.finish(m, info).cpu().numpy()
             if stage == "probe":
                 res[key]["force"] = lowered_force(v, 2e-6)
+                res[key]["probe_iters"] = m.conv.iters
                 pending.append((key, "real"))
             else:
                 res[key]["permeability"] = info["permeability"]
