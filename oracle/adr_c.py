@@ -107,6 +107,8 @@ def stepper(S):
     N, nd, ns = S.mf.size, S.nd, S.ns
     if N >= 2 ** 31:
         raise ValueError("grid too large for the 32-bit voxel indices")
+    if ns > 8:
+        raise ValueError("the C transport step holds at most 8 chemicals per voxel")
     mov = np.asarray(S.moving, np.int32)
     pore = np.flatnonzero(S.mf)
     n = np.arange(N)
