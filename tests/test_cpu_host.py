@@ -439,3 +439,22 @@ def test_slab_count_for_grids_beyond_the_32_bit_index():
         assert L.prt_lbm_pop_elems(q, code, *slab_dims(shape, widest)) > 0
         fewer = -(-shape[0] // (p - 1))
         assert L.prt_lbm_pop_elems(q, code, *slab_dims(shape, fewer)) < 0
+
+
+def test_host_campaign_estimate_units():
+    """bench.cpu_campaign_estimate: node iterations over the whole-host
+    D3Q19 rate plus pore-voxel steps over the per-core transport rate times
+    the processes, and the GPU's batched rate over the host's."""
+    import bench
+    campaign = {"n_solves": 64,
+                "work": {"stokes_node_iterations": 3_600_000_000,
+                         "transport_pore_voxel_steps": 7_200_000_000},
+                "stages": {"total": {"batched_solves_per_hour": 3600.0}}}
+    cpu = {"all_cores_procs": 10, "d3q19_64cube_mlups_all_cores": 10.0,
+           "adr_3d_64cube_sorption_pore_voxel_steps_per_s_1core": 2_000_000.0}
+    est = bench.cpu_campaign_estimate(campaign, cpu)
+    assert est["stokes_s"] == 360.0          # 3.6e9 / 10e6
+    assert est["transport_s"] == 360.0       # 7.2e9 / (2e6 * 10)
+    assert est["total_h"] == 0.2
+    assert est["solves_per_hour"] == 320.0   # 64 solves in 720 s
+    assert est["gpu_batched_over_host"] == 11.2
