@@ -841,7 +841,7 @@ def main():
         E = args.grid
         g = prtlb.sphere_pack((E, E, E), (4.0, 8.0), 0.35, seed=0)
         s = prtlb.StokesSolver(g, force=2e-6, storage=args.storage)
-        res = autotune.tune(s, steps=5, force=True)
+        res = autotune.tune(s, steps=5, force=True, passes=1)
         print(json.dumps({"shapes": {"grid": E, "storage": args.storage, "best": res["tune"],
                                      "order": res["order"], "ms": res["table"]}}), flush=True)
         return

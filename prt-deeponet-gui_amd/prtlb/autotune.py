@@ -73,31 +73,34 @@ def time_shape(s, tune, steps=10):
         s.tune = old
 
 
-def tune(s, steps=10, candidates=None, force=False):
+def tune(s, steps=10, candidates=None, force=False, passes=2):
     """Pick the fastest launch shape for StokesSolver `s`; sets s.tune and
-    returns {"tune": flags, "ms": ..., "table": {...}}.  Tuning advances the
-    solver's state by the iterations it times."""
+    returns {"tune": flags, "ms": ..., "table": {...}}.  Every shape is timed
+    in `passes` passes, the second in reverse order, and keeps its faster
+    time (a single short timing can be off by a few percent while the top
+    shapes lie within one).  Tuning advances the solver's state by the
+    iterations it times."""
     if not _CACHE:
         _load()
     key = _key(s)
     if key in _CACHE and not force:
         s.tune = _CACHE[key]["tune"]
         return _CACHE[key]
-    table = {"default": time_shape(s, 0, steps)}
-    order = [("default", 0)]                 # candidates in the order they ran
-    best, best_ms = 0, table["default"]
+    order = [("default", 0)]                 # candidates in the order they ran first
     for bz, by, npt, ntl, xcd in (candidates or CANDIDATES):
         if (1 << bz) * (1 << by) > 512 or ((ntl or xcd) and (s.flags or npt != 1)):
             continue
-        t = lbm_tune(bz, by, npt, bool(ntl), bool(xcd))
-        ms = time_shape(s, t, steps)
         name = "%dx%dx%d%s%s" % (1 << bz, 1 << by, npt, "nt" if ntl else "", "x" if xcd else "")
-        table[name] = ms
-        order.append((name, t))
-        if ms < best_ms:
-            best, best_ms = t, ms
+        order.append((name, lbm_tune(bz, by, npt, bool(ntl), bool(xcd))))
+    table = {}
+    for p in range(max(1, int(passes))):
+        for name, t in (order if p % 2 == 0 else order[::-1]):
+            ms = time_shape(s, t, steps)
+            table[name] = min(ms, table.get(name, ms))
+    best_name = min(table, key=lambda n: (table[n], n != "default"))
+    best = dict(order)[best_name]
     s.tune = best
-    _CACHE[key] = {"tune": best, "ms": best_ms, "table": table, "order": order}
+    _CACHE[key] = {"tune": best, "ms": table[best_name], "table": table, "order": order}
     _save()
     return _CACHE[key]
 
