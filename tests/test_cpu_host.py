@@ -458,3 +458,58 @@ def test_host_campaign_estimate_units():
     assert est["total_h"] == 0.2
     assert est["solves_per_hour"] == 320.0   # 64 solves in 720 s
     assert est["gpu_batched_over_host"] == 11.2
+
+
+def _adr_params_ok():
+    p = _lib.AdrParams()
+    p.nd, p.ns, p.dt = 3, 1, 0.1
+    p.abio_index = p.abio_target = p.abio_product = -1
+    p.mobile[0] = 1
+    return p
+
+
+@pytest.mark.parametrize("case", [
+    "q", "grid", "d2q9_nz", "lbm_range", "static_range", "table_nd", "adr_ns", "adr_dt", "adr_dface",
+    "adr_alias", "adr_steps", "adr_species_index"])
+def test_abi_refuses_bad_arguments_before_any_device_call(case):
+    """The C ABI's argument checks run on the host before any HIP call:
+    each bad argument returns its status (PRT_E_ARG -1, PRT_E_RANGE -3) and
+    a message, on a machine without a GPU."""
+    import ctypes
+    L = _lib.lib()
+    buf = (ctypes.c_double * 16)()
+    a, b, s = ctypes.addressof(buf), ctypes.addressof(buf) + 64, ctypes.addressof(buf) + 96
+    lbm = lambda q, nx, ny, nz, fn="prt_lbm_run": getattr(L, fn)(
+        q, 0, nx, ny, nz, s, a, b, 1, 1.25, 0.9, 1e-6, 1024.0, 0, None, None)
+    p = _adr_params_ok()
+    want, text = -1, None
+    if case == "q":
+        rc, text = lbm(7, 4, 4, 4), "q must be 9"
+    elif case == "grid":
+        rc, text = lbm(19, 4, 4, 0), "grid dimensions"
+    elif case == "d2q9_nz":
+        rc, text = lbm(9, 4, 4, 2), "D2Q9"
+    elif case == "lbm_range":
+        rc, want, text = lbm(19, 2000, 2000, 2000), -3, "32-bit"
+    elif case == "static_range":
+        rc, want = lbm(19, 2000, 2000, 2000, "prt_lbm_run_sum_static"), -3
+    elif case == "table_nd":
+        rc, text = L.prt_adr_pore_table(4, 4, 4, 4, s, a, b, 1, a, None), "nd must be 2 or 3"
+    else:
+        cin, cout, steps = a, b, 1
+        if case == "adr_ns":
+            p.ns, text = 9, "1..8 chemicals"
+        elif case == "adr_dt":
+            p.dt, text = 0.0, "dt must be positive"
+        elif case == "adr_dface":
+            p.dface[0][1], text = -1.0, "non-negative"
+        elif case == "adr_alias":
+            cout, text = a, "aliased"
+        elif case == "adr_steps":
+            steps, text = -1, "negative step count"
+        elif case == "adr_species_index":
+            p.abio_index, text = 3, "species index"
+        rc = L.prt_adr_run_list(ctypes.byref(p), 4, 4, 4, s, 4, cin, cout, steps, None)
+    assert rc == want, (case, rc)
+    msg = L.prt_last_error().decode()
+    assert msg and (text is None or text in msg), (case, msg)
