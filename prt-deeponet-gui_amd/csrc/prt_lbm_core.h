@@ -237,12 +237,14 @@ __device__ inline void gather_sparse(const LbmArgs &a, const L *__restrict__ lin
 // population itself), mixed per direction: the same state gives the same
 // bits, with no counter or generator state, so every launch path agrees.
 //
-// Odd multipliers, one per direction, that turn the node's seed into each
-// population's dither (the high 13 bits of the product).
+// Odd 24-bit multipliers, one per direction, that turn the node's seed into
+// each population's dither: bits 19-31 of the 24 x 24-bit product of the
+// seed's low 24 bits (significand bits) and the multiplier, one full-rate
+// multiply (a 32-bit integer multiply issues at a quarter of the rate).
 constexpr uint32_t kDitherMul[19] = {
-    0x9E3779B1u, 0x85EBCA77u, 0xC2B2AE3Du, 0x27D4EB2Fu, 0x165667B1u, 0xD3A2646Bu, 0xFD7046C5u,
-    0xB55A4F09u, 0x8F1BBCDDu, 0xCA62C1D7u, 0x5A827999u, 0x6ED9EBA1u, 0x3C6EF373u, 0xA54FF53Bu,
-    0x510E527Fu, 0x9B05688Du, 0x1F83D9ABu, 0x5BE0CD19u, 0xCBBB9D5Du};
+    0x9E3779u, 0x85EBCBu, 0xC2B2AFu, 0x27D4EBu, 0x165667u, 0xD3A265u, 0xFD7047u,
+    0xB55A4Fu, 0x8F1BBDu, 0xCA62C1u, 0x5A8279u, 0x6ED9EBu, 0x3C6EF3u, 0xA54FF5u,
+    0x510E53u, 0x9B0569u, 0x1F83D9u, 0x5BE0CDu, 0xCBBB9Du};
 template <int Q>
 __device__ inline uint32_t f16_dither_seed(const float *fp, int i) {
   return __float_as_uint(fp[i == 0 ? 1 : 0]);
@@ -253,7 +255,8 @@ __device__ inline uint32_t f16_dither_seed(const float *fp, int i) {
 // quiet NaNs keep their bits (their low 13 bits are zero, and the add does
 // not reach the exponent).
 __device__ inline unsigned short f16_round_dithered(float x, uint32_t seed, int i) {
-  const uint32_t d = (seed * kDitherMul[i]) >> 19;
+  // __umul24 returns a signed int: the shift must see the product unsigned
+  const uint32_t d = static_cast<uint32_t>(__umul24(seed, kDitherMul[i])) >> 19;
   return __half_as_ushort(__float2half_rn(__uint_as_float((__float_as_uint(x) + d) & ~0x1fffu)));
 }
 
