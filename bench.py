@@ -14,7 +14,11 @@ the fluid lists, the fused transport step, the batched 64-rock campaign
 (solves/hour batched FP32 against one-at-a-time FP64, permeability error),
 the 32-request serving run (solves/hour, p50/p99 latency, converged share)
 and the CPU baseline (this repo's NumPy oracle on the host, one core, on a
-bounded sample of the same 256^3 step).  Prints ONE JSON line.
+bounded sample of the same 256^3 step), with the host's per-core rates and
+its solves/hour for the 64-rock campaign estimated from them.  Prints ONE
+JSON line.  Studies run as separate invocations (--precision, --campaign N,
+--serve N [--serve-baseline], --adr-only, --fluid-list, --shapes, --tail,
+--slab, --big N, --geometry, --features).
 """
 import argparse
 import json
