@@ -119,3 +119,29 @@ def test_campaign_transport_batch_against_oracle():
         print("rock", k, geoms[k].shape, flush=True)
         assert np.array_equal(got[k][0], outo), k
         assert np.array_equal(got[k][1], tno), k
+
+
+@pytest.mark.timeout(900)
+def test_served_request_settles_at_the_oracles_step():
+    """A serving-mix request that settles before the 100 000-step cap
+    (request 29 of the 32-request draw: a 116^3 sphere pack, Pe 59.9, Da
+    1.76, reversible sorption, 763 636 pore voxels): served, its transport
+    stops at the step the oracle's own stop rule picks (NumPy-order pore
+    means every check), with every snapshot and t_norm bit for bit."""
+    pool = serve.poisson_requests(32, seed=0)
+    r = next(q for q in pool if q.key == 29)
+    assert r.g.shape == (116, 116, 116)
+    with Server() as S:
+        got = S.run([Request(r.key, r.g, r.pe, r.da, arrival=0.0)], realtime=False)[0]
+    info = got.result["info"]
+    print("served: %d steps, converged %s" % (info["steps"], info["converged"]), flush=True)
+    assert info["converged"] and info["steps"] < 100000
+    akw = C._adr_kw(dict(da=r.da), r.kinetics)
+    oinfo = {}
+    outo, tno = O.solve_adr(r.g, got.result["velocity"], r.pe, akw["da"], 5, n_species=2,
+                            max_steps=100000, chem=oracle_chem(akw["chem"]), info=oinfo,
+                            sorption=akw["sorption"], stepper=AC.stepper)
+    print("oracle: %d steps, converged %s" % (oinfo["steps"], oinfo["converged"]), flush=True)
+    assert oinfo["steps"] == info["steps"] and oinfo["converged"]
+    assert np.array_equal(got.result["out"], outo)
+    assert np.array_equal(got.result["t_norm"], tno)
