@@ -145,3 +145,37 @@ def test_served_request_settles_at_the_oracles_step():
     assert oinfo["steps"] == info["steps"] and oinfo["converged"]
     assert np.array_equal(got.result["out"], outo)
     assert np.array_equal(got.result["t_norm"], tno)
+
+
+@pytest.mark.timeout(900)
+def test_campaign_smallest_rocks_full_length_against_oracle():
+    """The two smallest rocks of the 64-rock draw at the campaign's full
+    length: irreversible sorption in automatic mode, which never settles,
+    so both run to the 100 000-step cap in one transport batch; snapshots,
+    t_norm and the step count equal the oracle's bit for bit."""
+    specs = sorted(C.campaign_specs(n_geom=64, seed=0), key=lambda s: s["shape"][0])[:2]
+    geoms = {s["key"]: C.make_geometry(s) for s in specs}
+    vels = {k: prtlb.stokes_d3q19(g, nit=400, force=2e-6, tol=0.0, storage="fp32")
+            for k, g in geoms.items()}
+    ab = AdrBatch()
+    for sp in specs:
+        kw = C._adr_kw(sp, "sorption_irreversible")
+        ab.add(sp["key"], geoms[sp["key"]], vels[sp["key"]], sp["pe"], kw.pop("da"), 5,
+               max_steps=100000, **kw)
+    got, infos = {}, {}
+    while len(ab):
+        for m in ab.advance():
+            infos[m.key] = {}
+            got[m.key] = ab.finish(m, infos[m.key])
+    for sp in specs:
+        k = sp["key"]
+        kw = C._adr_kw(sp, "sorption_irreversible")
+        oinfo = {}
+        outo, tno = O.solve_adr(geoms[k], vels[k], sp["pe"], kw["da"], 5, n_species=2,
+                                max_steps=100000, chem=oracle_chem(kw["chem"]),
+                                da_abio=kw["da_abio"], info=oinfo, stepper=AC.stepper)
+        print("rock", k, geoms[k].shape, infos[k]["steps"], oinfo["steps"], flush=True)
+        assert infos[k]["steps"] == oinfo["steps"] == 100000
+        assert not infos[k]["converged"] and not oinfo["converged"]
+        assert np.array_equal(got[k][0], outo), k
+        assert np.array_equal(got[k][1], tno), k
