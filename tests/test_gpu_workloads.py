@@ -179,3 +179,26 @@ def test_campaign_smallest_rocks_full_length_against_oracle():
         assert not infos[k]["converged"] and not oinfo["converged"]
         assert np.array_equal(got[k][0], outo), k
         assert np.array_equal(got[k][1], tno), k
+
+
+@pytest.mark.timeout(900)
+def test_served_2d_request_full_length_against_oracle():
+    """The smallest 2D request of the serving draw (a 284^2 disk pack,
+    reversible sorption) served alone through its whole transport, settled
+    or to the 100 000-step cap: the step count, snapshots and t_norm equal
+    the oracle's bit for bit."""
+    pool = serve.poisson_requests(32, seed=0)
+    r = min((q for q in pool if q.g.ndim == 2), key=lambda q: q.g.size)
+    with Server() as S:
+        got = S.run([Request(r.key, r.g, r.pe, r.da, arrival=0.0)], realtime=False)[0]
+    info = got.result["info"]
+    akw = C._adr_kw(dict(da=r.da), r.kinetics)
+    oinfo = {}
+    outo, tno = O.solve_adr(r.g, got.result["velocity"], r.pe, akw["da"], 5, n_species=2,
+                            max_steps=100000, chem=oracle_chem(akw["chem"]), info=oinfo,
+                            sorption=akw["sorption"], stepper=AC.stepper)
+    print("request", r.key, r.g.shape, "served", info["steps"], info["converged"], "oracle",
+          oinfo["steps"], oinfo["converged"], flush=True)
+    assert info["steps"] == oinfo["steps"] and info["converged"] == oinfo["converged"]
+    assert np.array_equal(got.result["out"], outo)
+    assert np.array_equal(got.result["t_norm"], tno)
