@@ -1,9 +1,11 @@
-"""The batched-campaign and serving workloads at their full sizes, shortened
-in time only: the 64-rock campaign (edges 48..160) and requests drawn from
-the serving mix (256^2..1024^2 disk packs, 64^3..256^3 sphere packs), each
-checked against the same solves run one at a time / served alone, and the
-transport fields of the largest members against the oracle (its steps
-computed in C, oracle/adr_c.c) bit for bit."""
+"""The batched-campaign and serving workloads at their full sizes: the
+64-rock campaign (edges 48..160) and requests drawn from the serving mix
+(256^2..1024^2 disk packs, 64^3..256^3 sphere packs).  Shortened runs check
+each member against the same solve run one at a time / served alone, and the
+largest members' transport fields against the oracle (its steps computed in
+C, oracle/adr_c.c) bit for bit; three members run their whole transport
+(settled, or to the 100 000-step cap) against the oracle, step count
+included."""
 import numpy as np
 import pytest
 
