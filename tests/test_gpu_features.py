@@ -2,7 +2,11 @@
 oracle (scipy spsolve of the same system): the float32 field within 2e-6
 (CG to a relative residual of 1e-12 against a direct solve), the dropped
 count exactly, and the gradient bit for bit when both start from the same
-field."""
+field.  Besides generated pore maps, the cases include the reference's
+released Monod domain, read as its own feature comparison reads it.
+The flow descriptors must match bit for bit."""
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -14,6 +18,20 @@ from prtlb import features
 pytestmark = pytest.mark.gpu
 
 
+def _released_domain():
+    """The reference's released 2D Monod domain (tests/golden/
+    reference_domains.npz), read as the reference's feature comparison
+    reads a released domain (3D/model/test_reference_parity.py:138-150):
+    a (64, 148) pore map with its top and bottom rows blocked, transposed
+    so that flow runs along axis 0."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "reference_domains.npz")
+    with np.load(path, allow_pickle=False) as z:
+        m = z["monod"].reshape(64, 148) > 0
+    m[0] = m[-1] = False
+    return np.ascontiguousarray(m.T)
+
+
 def _cases():
     rng = np.random.default_rng(0)
     p2 = rng.random((64, 40)) > 0.3
@@ -21,7 +39,8 @@ def _cases():
     p3 = prtlb.sphere_pack((40, 32, 28), (3.0, 6.0), 0.45, seed=2, connect=False) == prtlb.PORE
     p3[0] = p3[-1] = True
     slab = np.ones((21, 9), bool)
-    return [("random_2d", p2), ("spheres_3d", p3), ("slab_2d", slab)]
+    return [("random_2d", p2), ("spheres_3d", p3), ("slab_2d", slab),
+            ("released_monod_2d", _released_domain())]
 
 
 @pytest.mark.parametrize("name,pore", _cases(), ids=[c[0] for c in _cases()])
@@ -72,7 +91,8 @@ def _desc_cases():
     p3 = prtlb.sphere_pack((36, 30, 26), (3.0, 6.0), 0.45, seed=4) == prtlb.PORE
     chan = np.zeros((40, 9), bool)
     chan[:, 1:8] = True
-    return [("random_2d", p2), ("spheres_3d", p3), ("channel_2d", chan)]
+    return [("random_2d", p2), ("spheres_3d", p3), ("channel_2d", chan),
+            ("released_monod_2d", _released_domain())]
 
 
 @pytest.mark.parametrize("name,pore", _desc_cases(), ids=[c[0] for c in _desc_cases()])
