@@ -121,25 +121,37 @@ def _adr_kw(spec, kinetics):
 def physics_view(out, chem, kinetics):
     """(snapshots, chemistry) to hand check_physics for a run of `kinetics`.
 
-    check_physics gives each index its role (3D/tools/prtlb_3d.py:1684-1717):
+    check_physics gives each index its role (3D/tools/prtlb_3d.py:1684-1731):
     0 the donor, capped at its feed; 1 the acceptor, capped at its own feed;
-    2 the product, capped at its feed plus the donor's.  The sorption
-    kinetics (build-defined) keep the sorbed S at index 1, where that rule
-    would cap it at zero.  Their snapshots are checked with S in the product
-    slot and an unfed, empty acceptor between, so every rule applies with
-    the bound S really has: the mobile chemical it is made from."""
+    2 the product, capped at its feed plus the donor's; 3 the biomass, held
+    below a thousand times its start.  None of these bounds the sorbed S of
+    the sorption kinetics (build-defined, index 1): an immobile chemical
+    that keeps what the passing solute leaves behind, so irreversibly it
+    grows with time without bound (at Da 740 it reaches hundreds of times
+    the feed) and reversibly it tends to k_sorb / k_des times the solute.
+    The view therefore holds the mobile solute alone, under every rule of
+    the donor; sorbed_complaints checks S."""
     if not kinetics.startswith("sorption"):
         return out, chem
-    out = np.asarray(out)
-    empty = np.zeros_like(out[:, :1])
-    view = np.concatenate([out[:, :1], empty, out[:, 1:2]], axis=1)
-    ch = chemmod.Chemistry([chem.names[0], "(none)", chem.names[1]],
-                           [chem.inlet[0], np.nan, chem.inlet[1]],
-                           [chem.c_init[0], 0.0, chem.c_init[1]], [chem.d_rel[0], 0.0, 0.0],
-                           [False, True, True], biotic_on=False, abiotic_index=0,
-                           abiotic_product_index=2,
-                           abiotic_surface_only=chem.abiotic_surface_only)
-    return view, ch
+    return np.asarray(out)[:, :1], chem.truncate(1)
+
+
+def sorbed_complaints(out, mat, kinetics):
+    """The rules that do hold for the sorbed S of a sorption run (index 1
+    of the snapshots), in check_physics' wording (3D/tools/prtlb_3d.py:
+    1717-1737): finite, and not negative in the pore.  An empty list for
+    other kinetics."""
+    if not kinetics.startswith("sorption"):
+        return []
+    s = np.asarray(out)[:, 1][:, np.asarray(mat) == PORE]
+    bad = []
+    fin = np.isfinite(s)
+    if not fin.all():
+        bad.append("S is not finite: %d NaN and %d infinite values -- the integration "
+                   "diverged" % (int(np.isnan(s).sum()), int(np.isinf(s).sum())))
+    elif s.size and float(s.min()) < -1e-6:
+        bad.append("S reaches %.3g, which is negative" % float(s.min()))
+    return bad
 
 
 def run_sequential(specs, geoms, storage="fp64", tau=0.8, tol=1e-6, nit=30000,

@@ -157,6 +157,7 @@ def build_dataset(n_geom=4, shape=(48, 48, 48), n_sets=2, n_times=5, porosity=(0
         view, vchem = C.physics_view(out, chem, kinetics)
         bad = check_physics_device(view, mats[i], chem=vchem, da=kwk["da"],
                                    da_abio=kwk.get("da_abio"))
+        bad = bad + C.sorbed_complaints(out, mats[i], kinetics)
         for msg in bad:
             print("     PHYSICS run %d (Pe=%.2f Da_bio=%.2f Da_abio=%.2f): %s"
                   % (k, p, kwk["da"], kwk.get("da_abio", 0.0), msg), flush=True)

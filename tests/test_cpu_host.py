@@ -335,9 +335,11 @@ def test_chemistry_matches_the_oracle():
 
 def test_sorption_physics_view():
     """Irreversible sorption keeps the sorbed S at index 1, the acceptor's
-    slot; checked through campaign.physics_view S is held to the bound it
-    really has (the fed solute), so a sound run is quiet and an S above
-    the feed is still named."""
+    slot, where check_physics would cap it at zero.  S is immobile and keeps
+    what the passing solute leaves, so it may exceed the feed: through
+    campaign.physics_view the mobile solute alone meets the donor's rules
+    (a solute above its feed is still named), and sorbed_complaints holds S
+    to what does bound it (finite, not negative)."""
     from prtlb import campaign as C
     ch = prtlb.chemistry.sorption_irreversible()
     mat = np.full((30, 6), prtlb.PORE, np.uint8)
@@ -350,12 +352,25 @@ def test_sorption_physics_view():
     raw = prtlb.check_physics(out, mat, chem=ch, da_abio=1.0)
     assert any("S reaches" in s for s in raw)          # the acceptor's cap of zero
     view, vch = C.physics_view(out, ch, "sorption_irreversible")
-    assert view.shape == (3, 3, 30, 6) and list(vch.names) == ["C", "(none)", "S"]
+    assert view.shape == (3, 1, 30, 6) and list(vch.names) == ["C"]
     assert prtlb.check_physics(view, mat, chem=vch, da_abio=1.0) == []
-    out[2, 1, 5, 3] = 1.5
+    assert C.sorbed_complaints(out, mat, "sorption_irreversible") == []
+    out[2, 1, 5, 3] = 40.0                             # S well above the feed: sound
+    view, vch = C.physics_view(out, ch, "sorption_irreversible")
+    assert prtlb.check_physics(view, mat, chem=vch, da_abio=1.0) == []
+    assert C.sorbed_complaints(out, mat, "sorption_irreversible") == []
+    out[2, 1, 5, 3] = -0.5
+    assert C.sorbed_complaints(out, mat, "sorption_irreversible") == [
+        "S reaches -0.5, which is negative"]
+    out[2, 1, 5, 3] = np.nan
+    bad = C.sorbed_complaints(out, mat, "sorption_reversible")
+    assert len(bad) == 1 and "S is not finite: 1 NaN" in bad[0]
+    assert C.sorbed_complaints(out, mat, "monod") == []
+    out[2, 1, 5, 3] = 0.0
+    out[2, 0, 7, 2] = 1.5                              # the solute above its feed
     view, vch = C.physics_view(out, ch, "sorption_irreversible")
     bad = prtlb.check_physics(view, mat, chem=vch, da_abio=1.0)
-    assert len(bad) == 1 and "S reaches 1.500, above the 1.000" in bad[0]
+    assert len(bad) == 1 and "C reaches 1.500, above the 1.000" in bad[0]
 
 
 def test_batch_shapes_and_plans_host_side():

@@ -100,7 +100,8 @@ def test_reference_domain_pipeline_against_oracle(case):
     the others reach the cap), five snapshots -- the stop step,
     snapshots and t_norm bit for bit with the oracle (its steps computed
     in C, oracle/adr_c.c), the mobile chemical's outlet concentration to
-    1e-10."""
+    1e-10; and the reference's physics checks find nothing (the sorbed
+    chemical under its own rules)."""
     kin, pe, da, sorption = CASES[case]
     g = reference_domain(case)
     print("domain", case, g.shape, "porosity %.4f" % (g == prtlb.PORE).mean(), flush=True)
@@ -134,3 +135,7 @@ def test_reference_domain_pipeline_against_oracle(case):
     oc, _ = O.outlet_concentration(Co, g)
     assert oc[0] > 0
     assert abs(float(ainfo["outlet_concentration"][0]) / oc[0] - 1) < 1e-10
+    view, vch = C.physics_view(out, ch, kin)
+    assert prtlb.check_physics_device(view, g, chem=vch, da=da_t,
+                                      da_abio=akw.get("da_abio")) == []
+    assert C.sorbed_complaints(out, g, kin) == []
